@@ -1,0 +1,167 @@
+/*
+ * warpmpc_hip.h — C ABI of the MI355X (gfx950) batched fixed-pattern QP core.
+ *
+ * Plain C: no HIP or torch types.  Device pointers are `float*` into device
+ * memory owned by the caller; `stream` is a hipStream_t passed as `void*`
+ * (NULL = the default stream).  Every entry point is asynchronous on `stream`,
+ * never synchronises the host, and returns WMPC_OK or an error code whose text
+ * `wmpc_last_error()` gives (per thread).
+ *
+ * Layout ("symbolic" / batch-major, the reference's transpose_work layout,
+ * warpmpc/jax_qdldl/warp_backend.py:1188-1191): an array of `width` entries per
+ * instance is stored as [width][ld], instance b of entry i at `ptr[i*ld + b]`,
+ * with ld >= batch.  Each nonzero of K, L and D is therefore one contiguous
+ * stream across instances.
+ *
+ * Numerical failures (zero pivots) are not errors: they produce inf/NaN in the
+ * outputs exactly as the reference kernels do (warp_backend.py:97, 169).
+ */
+#ifndef WARPMPC_HIP_H
+#define WARPMPC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WMPC_OK 0
+#define WMPC_ERR_INVALID 1 /* bad argument or inconsistent pattern */
+#define WMPC_ERR_HIP 2     /* a HIP runtime call failed */
+#define WMPC_ERR_ALLOC 3   /* device allocation failed */
+#define WMPC_ERR_UNSUPPORTED 4
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* wmpc_last_error(void);
+/* Library version string. */
+const char* wmpc_version(void);
+
+/*
+ * Ragged form of one QDLDL plan (the QDLDLPlan fields of
+ * warpmpc/jax_qdldl/types.py:11-53 with the padding of the program tables
+ * dropped; orders unchanged).  Host pointers; only read during plan creation.
+ */
+typedef struct wmpc_qdldl_pattern {
+  int32_t n, nnz_a, nnz_l;
+  const int32_t* perm_to_orig; /* [nnz_a] permuted value slot -> original slot */
+  const int32_t* diag_pos;     /* [n] permuted slot of the diagonal of column k */
+  const int32_t* init_ptr;     /* [n+1] */
+  const int32_t* init_rows;    /* off-diagonal rows of permuted column k */
+  const int32_t* init_pos;     /* their permuted value slots */
+  const int32_t* proc_ptr;     /* [n+1] */
+  const int32_t* proc_cols;    /* reach of row k, processing order */
+  const int32_t* proc_lidx;    /* slot of L[k, c] */
+  const int32_t* proc_off;     /* entries column c holds before row k */
+  const int32_t* lp;           /* [n+1] L column pointers */
+  const int32_t* li;           /* [nnz_l] L row indices */
+  const int32_t* p;            /* [n] permutation */
+  const int32_t* pinv;         /* [n] inverse permutation */
+} wmpc_qdldl_pattern;
+
+typedef struct wmpc_qdldl_plan wmpc_qdldl_plan; /* opaque, device-resident tables */
+
+/* Replaces the table upload of compile_qdldl_warp (warp_backend.py:1230-1286).
+ * Validates the pattern, derives the kernels' tables and uploads them to the
+ * current device. */
+int wmpc_qdldl_plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out);
+int wmpc_qdldl_plan_destroy(wmpc_qdldl_plan* plan);
+/* Pattern sizes of a plan: n, nnz_a, nnz_l, max reach width. */
+int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a, int32_t* nnz_l,
+                         int32_t* max_reach);
+
+/* CompiledQDLDL.factor (warp_backend.py:1302-1349; kernel :104-173):
+ * a_values [nnz_a][ld] (original upper-CSC order) -> lx [nnz_l][ld],
+ * d, dinv [n][ld] (permuted order). */
+int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, float* lx, float* d,
+                          float* dinv, int64_t batch, int64_t ld, void* stream);
+
+/* CompiledQDLDL.solve (warp_backend.py:1351-1402; kernels :176-278):
+ * x = Pᵀ L⁻ᵀ D⁻¹ L⁻¹ P rhs.  rhs, x [n][ld] in original order.
+ * lx_ld / d_ld are the instance strides of the factor: ld for per-instance
+ * factors, 0 for one factor shared by the whole batch.  work: [n][ld] scratch. */
+int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t lx_ld,
+                         const float* dinv, int64_t d_ld, const float* rhs, float* x, float* work,
+                         int64_t batch, int64_t ld, void* stream);
+
+/* CompiledQDLDL.factor_and_solve (warp_backend.py:1404-1408), one launch:
+ * factor with the forward substitution fused into each row, then the backward
+ * sweep.  Outputs x, lx, d, dinv; work: [n][ld] scratch. */
+int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_values,
+                                const float* rhs, float* x, float* lx, float* d, float* dinv,
+                                float* work, int64_t batch, int64_t ld, void* stream);
+
+/*
+ * Fixed-pattern OSQP (warpmpc/jax_osqp/solver.py:685-982).  The descriptor
+ * carries the FixedOSQPPlan arrays (types.py:86-132) needed on the device.
+ */
+typedef struct wmpc_osqp_desc {
+  int32_t n, m, nnz_p, nnz_a, nnz_kkt;
+  int32_t nnz_p_full;
+  double sigma, alpha;
+  const double* kkt_base;      /* [nnz_kkt] KKT values in original upper-CSC order */
+  const int32_t* p_to_kkt;     /* [nnz_p] */
+  const double* p_diag_sigma;  /* [nnz_p] */
+  const int32_t* a_to_kkt;     /* [nnz_a] */
+  const double* p_scale;       /* [nnz_p] */
+  const double* a_scale;       /* [nnz_a] */
+  const double* q_scale;       /* [n] */
+  const double* scaling_d;     /* [n] */
+  const double* scaling_dinv;  /* [n] */
+  const double* scaling_e;     /* [m] */
+  const double* scaling_einv;  /* [m] */
+  const double* y_scale;       /* [m] */
+  const double* y_unscale;     /* [m] */
+  const double* rho_vec;       /* [m] */
+  const double* rho_inv_vec;   /* [m] */
+  const int32_t* a_rows;       /* [nnz_a] */
+  const int32_t* a_cols;       /* [nnz_a] */
+  const int32_t* p_full_rows;  /* [nnz_p_full] */
+  const int32_t* p_full_cols;  /* [nnz_p_full] */
+  const int32_t* p_full_value_map; /* [nnz_p_full] */
+} wmpc_osqp_desc;
+
+typedef struct wmpc_osqp_plan wmpc_osqp_plan;
+
+/* Replaces the constant setup of compile_osqp (solver.py:739-762).  `kkt` must
+ * be the plan of desc's KKT pattern; it is referenced, not copied, and must
+ * outlive the OSQP plan. */
+int wmpc_osqp_plan_create(const wmpc_osqp_desc* desc, const wmpc_qdldl_plan* kkt,
+                          wmpc_osqp_plan** out);
+int wmpc_osqp_plan_destroy(wmpc_osqp_plan* plan);
+
+/* CompiledOSQP.factor (solver.py:799-804): KKT value scatter + factorization.
+ * p_values [nnz_p][pm_ld], a_values [nnz_a][pm_ld]; pm_ld = 0 means one matrix
+ * shared by all instances (batch must then be 1 for the factor).  Outputs
+ * lx [nnz_l][ld], dinv [n_kkt][ld] (d is not returned, as in the reference). */
+int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
+                         const float* a_values, int64_t pm_ld, float* lx, float* dinv,
+                         int64_t batch, int64_t ld, void* stream);
+
+/*
+ * CompiledOSQP.solve / solve_warm_start / solve_with_factor[_warm_start]
+ * (solver.py:822-982) fused into one persistent launch: per instance the KKT
+ * factor (unless `lx` is given), max_iter ADMM iterations (rhs build, solve,
+ * relaxation, projection, dual update), unscaling, residuals and objective.
+ *
+ * p_values/a_values: [nnz][pm_ld], pm_ld = 0 when shared by the batch.
+ * q [n][ld], l, u [m][ld]; x0 [n][ld], z0, y0 [m][ld] may be NULL (cold start).
+ * lx/dinv: if non-NULL, a precomputed factor with instance stride f_ld
+ * (0 = shared) and no factorization is done; otherwise the factor is computed
+ * into the caller's lx_work/dinv_work ([nnz_l][ld], [n_kkt][ld]) — for a shared
+ * matrix (pm_ld = 0) only instance 0 of the work buffers is used.
+ * Outputs: x [n][ld], z, y [m][ld], prim, dual, obj [ld].
+ * work: [n_kkt][ld] scratch.
+ */
+int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
+                        const float* a_values, int64_t pm_ld, const float* q, const float* l,
+                        const float* u, const float* x0, const float* z0, const float* y0,
+                        const float* lx, const float* dinv, int64_t f_ld, float* lx_work,
+                        float* dinv_work, float* work, float* x, float* z, float* y, float* prim,
+                        float* dual, float* obj, int32_t max_iter, int64_t batch, int64_t ld,
+                        void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WARPMPC_HIP_H */

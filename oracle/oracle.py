@@ -1,0 +1,269 @@
+"""ctypes front end of the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` as the checker.  Product code never imports
+this module.
+
+Array conventions are instance-major ``[batch, width]`` (C-contiguous), i.e. the
+reference's "batch" layout; callers transpose symbolic-layout data themselves.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+
+class _Pattern(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("nnz_l", ctypes.c_int32),
+        ("perm_to_orig", _i32p),
+        ("diag_pos", _i32p),
+        ("init_ptr", _i32p),
+        ("init_rows", _i32p),
+        ("init_pos", _i32p),
+        ("proc_ptr", _i32p),
+        ("proc_cols", _i32p),
+        ("proc_lidx", _i32p),
+        ("proc_off", _i32p),
+        ("lp", _i32p),
+        ("li", _i32p),
+        ("p", _i32p),
+        ("pinv", _i32p),
+    ]
+
+
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class _OsqpDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("nnz_p", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("nnz_kkt", ctypes.c_int32),
+        ("max_iter", ctypes.c_int32),
+        ("nnz_p_full", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+        ("sigma", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("kkt", ctypes.POINTER(_Pattern)),
+        ("kkt_base", _f64p),
+        ("p_to_kkt", _i32p),
+        ("p_diag_sigma", _f64p),
+        ("a_to_kkt", _i32p),
+        ("p_scale", _f64p),
+        ("a_scale", _f64p),
+        ("q_scale", _f64p),
+        ("scaling_d", _f64p),
+        ("scaling_dinv", _f64p),
+        ("scaling_e", _f64p),
+        ("scaling_einv", _f64p),
+        ("y_scale", _f64p),
+        ("y_unscale", _f64p),
+        ("rho_vec", _f64p),
+        ("rho_inv_vec", _f64p),
+        ("a_rows", _i32p),
+        ("a_cols", _i32p),
+        ("p_full_rows", _i32p),
+        ("p_full_cols", _i32p),
+        ("p_full_value_map", _i32p),
+    ]
+
+
+def build() -> pathlib.Path:
+    """Compile ``liboracle.so`` with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
+    return _HERE / "liboracle.so"
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            build()
+        _LIB = ctypes.CDLL(str(path))
+        vp = ctypes.c_void_p
+        i64 = ctypes.c_int64
+        for sfx in ("f64", "f32"):
+            getattr(_LIB, f"oracle_qdldl_factor_{sfx}").argtypes = [ctypes.POINTER(_Pattern), i64, vp, vp, vp, vp]
+            getattr(_LIB, f"oracle_qdldl_solve_{sfx}").argtypes = [
+                ctypes.POINTER(_Pattern), i64, vp, i64, vp, i64, vp, vp]
+            getattr(_LIB, f"oracle_osqp_solve_{sfx}").argtypes = [
+                ctypes.POINTER(_OsqpDesc), i64, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        _LIB.oracle_num_threads.restype = ctypes.c_int
+    return _LIB
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OraclePattern:
+    """Keeps the int32 tables alive next to the ctypes struct that points at them."""
+
+    def __init__(self, plan):
+        from warpmpc_amd.qdldl.tables import ragged_program
+
+        self.plan = plan
+        self.tables = ragged_program(plan)
+        s = _Pattern()
+        s.n = plan.n
+        s.nnz_a = plan.nnz_a
+        s.nnz_l = plan.nnz_l
+        for name, arr in self.tables.items():
+            setattr(s, name, arr.ctypes.data_as(_i32p))
+        self.struct = s
+
+
+def _dtype_sfx(dtype) -> str:
+    dtype = np.dtype(dtype)
+    if dtype == np.float64:
+        return "f64"
+    if dtype == np.float32:
+        return "f32"
+    raise ValueError(f"oracle supports float32/float64, got {dtype}")
+
+
+def qdldl_factor(plan, a_values: np.ndarray, dtype=np.float64):
+    """Factor ``a_values[batch, nnz_a]`` -> ``(lx[batch, nnz_l], d[batch, n], dinv[batch, n])``."""
+    sfx = _dtype_sfx(dtype)
+    pat = OraclePattern(plan)
+    a = np.ascontiguousarray(np.atleast_2d(a_values), dtype=dtype)
+    b = a.shape[0]
+    lx = np.zeros((b, plan.nnz_l), dtype=dtype)
+    d = np.zeros((b, plan.n), dtype=dtype)
+    dinv = np.zeros((b, plan.n), dtype=dtype)
+    rc = getattr(lib(), f"oracle_qdldl_factor_{sfx}")(ctypes.byref(pat.struct), b, _ptr(a), _ptr(lx), _ptr(d), _ptr(dinv))
+    if rc:
+        raise RuntimeError("oracle factor failed")
+    return lx, d, dinv
+
+
+def qdldl_solve(plan, lx: np.ndarray, dinv: np.ndarray, rhs: np.ndarray, dtype=np.float64):
+    """Solve with factors ``lx[b or 1, nnz_l]``, ``dinv[b or 1, n]`` for ``rhs[batch, n]``."""
+    sfx = _dtype_sfx(dtype)
+    pat = OraclePattern(plan)
+    lx = np.ascontiguousarray(np.atleast_2d(lx), dtype=dtype)
+    dinv = np.ascontiguousarray(np.atleast_2d(dinv), dtype=dtype)
+    rhs = np.ascontiguousarray(np.atleast_2d(rhs), dtype=dtype)
+    b = rhs.shape[0]
+    out = np.zeros_like(rhs)
+    lxs = 0 if lx.shape[0] == 1 else plan.nnz_l
+    ds = 0 if dinv.shape[0] == 1 else plan.n
+    rc = getattr(lib(), f"oracle_qdldl_solve_{sfx}")(
+        ctypes.byref(pat.struct), b, _ptr(lx), lxs, _ptr(dinv), ds, _ptr(rhs), _ptr(out))
+    if rc:
+        raise RuntimeError("oracle solve failed")
+    return out
+
+
+class OracleOSQP:
+    """Fixed-iteration OSQP oracle bound to one :class:`FixedOSQPPlan`."""
+
+    def __init__(self, plan, max_iter: int | None = None):
+        self.plan = plan
+        self.pattern = OraclePattern(plan.qdldl_plan)
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        self.keep = {
+            "kkt_base": f64(plan.kkt_upper.data),
+            "p_to_kkt": i32(plan.p_to_kkt),
+            "p_diag_sigma": f64(plan.p_diag_sigma),
+            "a_to_kkt": i32(plan.a_to_kkt),
+            "p_scale": f64(plan.p_scale),
+            "a_scale": f64(plan.a_scale),
+            "q_scale": f64(plan.q_scale),
+            "scaling_d": f64(plan.scaling_d),
+            "scaling_dinv": f64(plan.scaling_dinv),
+            "scaling_e": f64(plan.scaling_e),
+            "scaling_einv": f64(plan.scaling_einv),
+            "y_scale": f64(plan.y_scale),
+            "y_unscale": f64(plan.y_unscale),
+            "rho_vec": f64(plan.rho_vec),
+            "rho_inv_vec": f64(plan.rho_inv_vec),
+            "a_rows": i32(plan.a_rows),
+            "a_cols": i32(plan.a_cols),
+            "p_full_rows": i32(plan.p_full_rows),
+            "p_full_cols": i32(plan.p_full_cols),
+            "p_full_value_map": i32(plan.p_full_value_map),
+        }
+        d = _OsqpDesc()
+        d.n, d.m = plan.n, plan.m
+        d.nnz_p, d.nnz_a, d.nnz_kkt = plan.nnz_p, plan.nnz_a, plan.nnz_kkt
+        d.max_iter = plan.settings.max_iter if max_iter is None else int(max_iter)
+        d.nnz_p_full = int(plan.p_full_rows.size)
+        d.sigma = plan.settings.sigma
+        d.alpha = plan.settings.alpha
+        d.kkt = ctypes.pointer(self.pattern.struct)
+        for name, arr in self.keep.items():
+            ptype = _f64p if arr.dtype == np.float64 else _i32p
+            setattr(d, name, arr.ctypes.data_as(ptype))
+        self.desc = d
+
+    def solve(self, p_values, a_values, q, l, u, x0=None, z0=None, y0=None, dtype=np.float64):
+        """Returns ``(x, z, y, prim_res, dual_res, obj_val)`` like ``CompiledOSQP.solve``."""
+        sfx = _dtype_sfx(dtype)
+        plan = self.plan
+        pv = np.ascontiguousarray(np.atleast_2d(p_values), dtype=dtype)
+        av = np.ascontiguousarray(np.atleast_2d(a_values), dtype=dtype)
+        q = np.ascontiguousarray(np.atleast_2d(q), dtype=dtype)
+        l = np.ascontiguousarray(np.atleast_2d(l), dtype=dtype)
+        u = np.ascontiguousarray(np.atleast_2d(u), dtype=dtype)
+        b = max(pv.shape[0], av.shape[0], q.shape[0], l.shape[0], u.shape[0])
+        q = np.ascontiguousarray(np.broadcast_to(q, (b, plan.n)))
+        l = np.ascontiguousarray(np.broadcast_to(l, (b, plan.m)))
+        u = np.ascontiguousarray(np.broadcast_to(u, (b, plan.m)))
+        shared = pv.shape[0] == 1 and av.shape[0] == 1
+        if not shared:
+            pv = np.ascontiguousarray(np.broadcast_to(pv, (b, plan.nnz_p)))
+            av = np.ascontiguousarray(np.broadcast_to(av, (b, plan.nnz_a)))
+        ps = 0 if shared else plan.nnz_p
+        as_ = 0 if shared else plan.nnz_a
+
+        def opt(v, w):
+            if v is None:
+                return None, None
+            arr = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(np.asarray(v, dtype=dtype)), (b, w)))
+            return arr, _ptr(arr)
+
+        x0a, x0p = opt(x0, plan.n)
+        z0a, z0p = opt(z0, plan.m)
+        y0a, y0p = opt(y0, plan.m)
+        x = np.zeros((b, plan.n), dtype=dtype)
+        z = np.zeros((b, plan.m), dtype=dtype)
+        y = np.zeros((b, plan.m), dtype=dtype)
+        prim = np.zeros(b, dtype=dtype)
+        dual = np.zeros(b, dtype=dtype)
+        obj = np.zeros(b, dtype=dtype)
+        rc = getattr(lib(), f"oracle_osqp_solve_{sfx}")(
+            ctypes.byref(self.desc), b, _ptr(pv), ps, _ptr(av), as_, _ptr(q), _ptr(l), _ptr(u),
+            x0p, z0p, y0p, _ptr(x), _ptr(z), _ptr(y), _ptr(prim), _ptr(dual), _ptr(obj))
+        if rc:
+            raise RuntimeError("oracle osqp failed")
+        del x0a, z0a, y0a
+        return x, z, y, prim, dual, obj
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def set_threads(n: int) -> None:
+    os.environ["OMP_NUM_THREADS"] = str(int(n))
+
+
+__all__ = ["OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor", "qdldl_solve"]

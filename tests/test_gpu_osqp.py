@@ -1,0 +1,117 @@
+"""GPU parity of the fused HIP OSQP solve against the FP64 ADMM oracle.
+
+Error measure: per instance and output, max |v - v64| / max(|v64|_inf, 1)
+(x, z, y are O(1) here; residuals and objective are compared absolutely with
+the same floor).  Tolerance 1e-4: 25 float32 ADMM iterations vs the FP64
+restatement of solver.py:806-872 on the same inputs.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle
+from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
+from warpmpc_amd.problems import batched_mpc_data, mass_chain_mpc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _err(v, ref):
+    v = np.atleast_2d(np.asarray(v, dtype=np.float64))
+    ref = np.atleast_2d(np.asarray(ref, dtype=np.float64))
+    if v.shape[0] == 1 and ref.shape[0] != 1:
+        v = v.T
+        ref = ref.T
+    return float(np.max(np.max(np.abs(v - ref), axis=-1) / np.maximum(np.max(np.abs(ref), axis=-1), 1.0)))
+
+
+def _check(out, ref, tol=TOL):
+    names = ("x", "z", "y", "prim_res", "dual_res", "obj_val")
+    for name, o, r in zip(names, out, ref):
+        o = o.cpu().numpy()
+        if o.ndim == 1:
+            e = float(np.max(np.abs(o - r) / np.maximum(np.abs(r), 1.0)))
+        else:
+            e = _err(o, r)
+        assert e < tol, f"{name}: {e}"
+
+
+@pytest.fixture(scope="module")
+def chain(gpu):
+    prob = mass_chain_mpc(horizon=10)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
+    return prob, plan, compile_osqp(plan, np.float32, qdldl_backend="hip", transpose_work=True)
+
+
+def test_shared_matrix_solve_matches_oracle(gpu, chain):
+    prob, plan, solver = chain
+    pv, av, q, l, u = batched_mpc_data(prob, 1000, seed=0)
+    out = solver.solve(pv, av, q, l, u)
+    ref = oracle.OracleOSQP(plan).solve(pv, av, q, l, u)
+    _check(out, ref)
+
+
+def test_per_instance_matrices_match_oracle(gpu, chain):
+    prob, plan, solver = chain
+    b = 300
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=5)
+    rng = np.random.default_rng(5)
+    pv = pv * rng.uniform(0.5, 2.0, size=(b, pv.shape[1]))
+    av = np.broadcast_to(av, (b, av.shape[1])).copy()
+    q = rng.normal(size=q.shape)
+    out = solver.solve(pv, av, q, l, u)
+    ref = oracle.OracleOSQP(plan).solve(pv, av, q, l, u)
+    _check(out, ref)
+
+
+def test_warm_start_and_factor_reuse(gpu, chain):
+    prob, plan, solver = chain
+    b = 257
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=2)
+    rng = np.random.default_rng(2)
+    ws = (rng.normal(size=(b, plan.n)), rng.normal(size=(b, plan.m)), rng.normal(size=(b, plan.m)))
+    out = solver.solve_warm_start(pv, av, q, l, u, *ws)
+    ref = oracle.OracleOSQP(plan).solve(pv, av, q, l, u, x0=ws[0], z0=ws[1], y0=ws[2])
+    _check(out, ref)
+    lx, dinv = solver.factor(pv, av)
+    assert lx.shape == (plan.qdldl_plan.nnz_l, 1)
+    direct = solver.solve(pv, av, q, l, u)
+    reused = solver.solve_with_factor(lx, dinv, pv, av, q, l, u)
+    for a, c in zip(direct, reused):
+        np.testing.assert_allclose(a.cpu().numpy(), c.cpu().numpy(), rtol=1e-6, atol=1e-6)
+
+
+def test_reference_fixture_scaled_problem(gpu):
+    """tests/test_jax_osqp.py:97-125 on the HIP backend (float32)."""
+    p = sp.csc_matrix([[1e-4, 0.0], [0.0, 1e4]])
+    a = sp.eye(2, format="csc")
+    l, u, q = np.array([0.0, 0.0]), np.array([10.0, 10.0]), np.array([-1.0, -1.0])
+    plan = build_osqp_plan(p, a, l, u, OSQPSettings(max_iter=200, scaling=10), scaling_q=q)
+    solver = compile_osqp(plan, np.float32, qdldl_backend="hip", transpose_work=True)
+    x, z, y, prim, dual, obj = solver.solve(plan.p_upper.data[None], plan.a_matrix.data[None], q[None],
+                                            l[None], u[None])
+    np.testing.assert_allclose(x.cpu().numpy(), [[10.0, 1e-4]], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(obj.cpu().numpy(), [-9.99505], rtol=1e-5)
+
+
+def test_minimal_osqp_example(gpu):
+    """examples/jax_osqp_minimal.py:21-66 shapes: per-instance q, batch 2."""
+    p = sp.csc_matrix([[4.0, 1.0], [1.0, 2.0]])
+    a = sp.eye(2, format="csc")
+    l, u = np.array([0.0, 0.0]), np.array([1.0, 1.0])
+    q = np.array([[-1.0, -1.0], [-2.0, -0.5]])
+    plan = build_osqp_plan(p, a, l, u, OSQPSettings(max_iter=50, scaling=0))
+    solver = compile_osqp(plan, np.float32, qdldl_backend="hip", transpose_work=True, segmented=True,
+                          segment_budget=8, level_scheduled_solve=True, level_scheduled_solve_threshold=2)
+    pv = np.broadcast_to(plan.p_upper.data, (2, plan.nnz_p))
+    av = np.broadcast_to(plan.a_matrix.data, (2, plan.nnz_a))
+    lb, ub = np.broadcast_to(l, q.shape), np.broadcast_to(u, q.shape)
+    out = solver.solve(pv, av, q, lb, ub)
+    ref = oracle.OracleOSQP(plan).solve(pv, av, q, lb, ub)
+    _check(out, ref)

@@ -1,0 +1,147 @@
+"""ctypes binding of the HIP library ``libwarpmpc_hip.so`` (C ABI: include/warpmpc_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+every entry point raises instead of falling back to anything else.
+Device memory and streams come from PyTorch-ROCm tensors (``data_ptr()`` and
+``torch.cuda.current_stream().cuda_stream``); PyTorch is plumbing here.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import pathlib
+import threading
+
+import numpy as np
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libwarpmpc_hip.so"
+
+_lock = threading.Lock()
+_lib: ctypes.CDLL | None = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+
+WMPC_OK = 0
+
+
+class QdldlPattern(ctypes.Structure):
+    """Mirror of ``wmpc_qdldl_pattern``."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("nnz_l", ctypes.c_int32),
+        ("perm_to_orig", _i32p),
+        ("diag_pos", _i32p),
+        ("init_ptr", _i32p),
+        ("init_rows", _i32p),
+        ("init_pos", _i32p),
+        ("proc_ptr", _i32p),
+        ("proc_cols", _i32p),
+        ("proc_lidx", _i32p),
+        ("proc_off", _i32p),
+        ("lp", _i32p),
+        ("li", _i32p),
+        ("p", _i32p),
+        ("pinv", _i32p),
+    ]
+
+
+class OsqpDesc(ctypes.Structure):
+    """Mirror of ``wmpc_osqp_desc``."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("nnz_p", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("nnz_kkt", ctypes.c_int32),
+        ("nnz_p_full", ctypes.c_int32),
+        ("sigma", ctypes.c_double),
+        ("alpha", ctypes.c_double),
+        ("kkt_base", _f64p),
+        ("p_to_kkt", _i32p),
+        ("p_diag_sigma", _f64p),
+        ("a_to_kkt", _i32p),
+        ("p_scale", _f64p),
+        ("a_scale", _f64p),
+        ("q_scale", _f64p),
+        ("scaling_d", _f64p),
+        ("scaling_dinv", _f64p),
+        ("scaling_e", _f64p),
+        ("scaling_einv", _f64p),
+        ("y_scale", _f64p),
+        ("y_unscale", _f64p),
+        ("rho_vec", _f64p),
+        ("rho_inv_vec", _f64p),
+        ("a_rows", _i32p),
+        ("a_cols", _i32p),
+        ("p_full_rows", _i32p),
+        ("p_full_cols", _i32p),
+        ("p_full_value_map", _i32p),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/warpmpc_hip.h declares
+SIGNATURES: dict[str, tuple] = {
+    "wmpc_last_error": (ctypes.c_char_p, []),
+    "wmpc_version": (ctypes.c_char_p, []),
+    "wmpc_qdldl_plan_create": (ctypes.c_int, [ctypes.POINTER(QdldlPattern), ctypes.POINTER(_vp)]),
+    "wmpc_qdldl_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_qdldl_plan_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
+    "wmpc_qdldl_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_qdldl_solve_f32": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_qdldl_factor_solve_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_osqp_plan_create": (ctypes.c_int, [ctypes.POINTER(OsqpDesc), _vp, ctypes.POINTER(_vp)]),
+    "wmpc_osqp_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_osqp_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_osqp_solve_f32": (ctypes.c_int, [
+        _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+        _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
+}
+
+
+class HipError(RuntimeError):
+    """A C-ABI call returned an error code."""
+
+
+def load(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
+    """Load (once) and type the HIP library; raises if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = pathlib.Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise ImportError(
+                f"{p} is missing: build the HIP library first (python -m warpmpc_amd.build)"
+            )
+        lib = ctypes.CDLL(str(p))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != WMPC_OK:
+        msg = load().wmpc_last_error().decode(errors="replace")
+        raise HipError(f"{what} failed (code {rc}): {msg}")
+
+
+def i32p(a: np.ndarray):
+    return a.ctypes.data_as(_i32p)
+
+
+def f64p(a: np.ndarray):
+    return a.ctypes.data_as(_f64p)
+
+
+__all__ = ["HipError", "LIB_PATH", "OsqpDesc", "QdldlPattern", "SIGNATURES", "check", "load"]

@@ -1,0 +1,15 @@
+"""Fixed-pattern batched OSQP (ADMM) driving the HIP QDLDL kernels."""
+
+from .planning import build_osqp_plan
+from .solver import compile_osqp
+from .types import CompiledOSQP, FixedOSQPPlan, OSQPDerivativePlan, OSQPSettings, OSQPWarmStart
+
+__all__ = [
+    "CompiledOSQP",
+    "FixedOSQPPlan",
+    "OSQPDerivativePlan",
+    "OSQPSettings",
+    "OSQPWarmStart",
+    "build_osqp_plan",
+    "compile_osqp",
+]
