@@ -326,7 +326,7 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
                          const float* a_values, int64_t pm_ld, float* lx, float* dinv,
                          int64_t batch, int64_t ld, void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && p_values && a_values && lx && dinv, "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && lx && dinv)), "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   WMPC_REQUIRE(pm_ld >= batch || (pm_ld == 0 && batch <= 1),
                "pm_ld must be >= batch, or 0 with batch 1 (one shared matrix, one factor)");
@@ -349,8 +349,7 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         float* dual, float* obj, int32_t max_iter, int64_t batch, int64_t ld,
                         void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && p_values && a_values && q && l && u && work && x && z && y && prim && dual && obj,
-               "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && q && l && u && work && x && z && y && prim && dual && obj)), "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   WMPC_REQUIRE(pm_ld == 0 || pm_ld >= batch, "pm_ld must be 0 (shared) or >= batch");
   WMPC_REQUIRE(max_iter >= 0, "max_iter must be nonnegative");

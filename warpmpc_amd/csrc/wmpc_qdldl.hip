@@ -233,7 +233,7 @@ int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a
 int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, float* lx, float* d,
                           float* dinv, int64_t batch, int64_t ld, void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && a_values && lx && d && dinv, "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (a_values && lx && d && dinv)), "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   if (batch == 0) return WMPC_OK;
   const int64_t grid = (batch + kBlock - 1) / kBlock;
@@ -247,7 +247,7 @@ int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t l
                          const float* dinv, int64_t d_ld, const float* rhs, float* x, float* work,
                          int64_t batch, int64_t ld, void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && lx && dinv && rhs && x && work, "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (lx && dinv && rhs && x && work)), "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   WMPC_REQUIRE((lx_ld == 0 && d_ld == 0) || (lx_ld >= batch && d_ld >= batch),
                "factor strides must both be 0 (shared) or both >= batch");
@@ -269,7 +269,7 @@ int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_valu
                                 const float* rhs, float* x, float* lx, float* d, float* dinv,
                                 float* work, int64_t batch, int64_t ld, void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && a_values && rhs && x && lx && d && dinv && work, "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (a_values && rhs && x && lx && d && dinv && work)), "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   if (batch == 0) return WMPC_OK;
   const int64_t grid = (batch + kBlock - 1) / kBlock;

@@ -160,4 +160,42 @@ def natural_order(n: int) -> np.ndarray:
     return np.arange(n, dtype=np.int32)
 
 
-__all__ = ["amd_order", "natural_order"]
+def etree_postorder(parent: np.ndarray) -> np.ndarray:
+    """Postorder of an elimination forest, larger subtrees first.
+
+    Returns ``post`` with ``post[i]`` = the node placed at position ``i``.  Any
+    postorder of the elimination tree is an equivalent ordering (same L pattern
+    up to relabelling, same fill).  Visiting the children of a node in order of
+    decreasing subtree size puts small subtrees (for a KKT matrix: the leaf
+    multipliers of box constraints) immediately before their parent, so a
+    column of L stays open - referenced by later rows - only for a short span
+    of rows.  This bounds the per-instance working set of the factorization by
+    the band of the matrix instead of by its size.  Ties go to the smaller index.
+    """
+    parent = np.asarray(parent)
+    n = parent.size
+    size = np.ones(n, dtype=np.int64)
+    for v in range(n):  # etree parents have larger indices than their children
+        if parent[v] >= 0:
+            size[parent[v]] += size[v]
+    children: list[list[int]] = [[] for _ in range(n)]
+    roots: list[int] = []
+    for v in range(n):
+        (children[parent[v]] if parent[v] >= 0 else roots).append(v)
+    for ch in children:
+        ch.sort(key=lambda c: (-int(size[c]), c))
+    roots.sort(key=lambda c: (-int(size[c]), c))
+    post: list[int] = []
+    for r in roots:
+        stack = [(r, 0)]
+        while stack:
+            v, i = stack.pop()
+            if i < len(children[v]):
+                stack.append((v, i + 1))
+                stack.append((children[v][i], 0))
+            else:
+                post.append(v)
+    return np.asarray(post, dtype=np.int32)
+
+
+__all__ = ["amd_order", "etree_postorder", "natural_order"]

@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 import scipy.sparse as sp
 
-from .ordering import amd_order, natural_order
+from .ordering import amd_order, etree_postorder, natural_order
 from .types import QDLDLPlan
 
 QDLDL_UNKNOWN = -1
@@ -235,6 +235,11 @@ def build_qdldl_plan(
         ordering = "external"
     elif ordering == "amd":
         p = amd_order(n, a_upper.indptr, a_upper.indices)
+        # relabel by a largest-subtree-first postorder of the elimination tree
+        # (same fill, short column lifetimes; see ordering.etree_postorder)
+        ap0, ai0, _ = symperm_upper(a_upper, inverse_permutation(p))
+        _, parent0, _ = qdldl_etree(n, ap0, ai0)
+        p = np.ascontiguousarray(p[etree_postorder(parent0)], dtype=np.int32)
     elif ordering == "natural":
         p = natural_order(n)
     else:
