@@ -46,7 +46,23 @@ def _parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-osqp", action="store_true", help="skip the OSQP forward leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return ap.parse_args()
+
+
+def _copy_calibration(steps: int) -> int:
+    """Plain device-to-device copy of 1 GiB: the known-size kernel that calibrates
+    the FETCH_SIZE / WRITE_SIZE counters (reads 1 GiB, writes 1 GiB per launch)."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": src.numel() * 4, "launches": steps}), flush=True)
+    return 0
 
 
 def _traffic_from_profile(name: str):
@@ -64,6 +80,8 @@ def main() -> int:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (instances shard with no "
               "communication, so N GPUs run N independent copies of this workload)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return _copy_calibration(args.steps)
     import numpy as np
     import torch
 
