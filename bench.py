@@ -109,7 +109,7 @@ def main() -> int:
     def step():
         _lib.check(lib.wmpc_qdldl_factor_solve_f32(dev.handle, vals.data_ptr(), rhs.data_ptr(), x.data_ptr(),
                                                    lx.data_ptr(), d.data_ptr(), dinv.data_ptr(), work.data_ptr(),
-                                                   b, b, stream.cuda_stream), "factor_solve")
+                                                   work.numel(), b, b, stream.cuda_stream), "factor_solve")
 
     print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):

@@ -58,16 +58,35 @@ typedef struct wmpc_qdldl_pattern {
   const int32_t* pinv;         /* [n] inverse permutation */
 } wmpc_qdldl_pattern;
 
-typedef struct wmpc_qdldl_plan wmpc_qdldl_plan; /* opaque, device-resident tables */
+typedef struct wmpc_qdldl_plan wmpc_qdldl_plan; /* opaque, device-resident programs */
 
 /* Replaces the table upload of compile_qdldl_warp (warp_backend.py:1230-1286).
- * Validates the pattern, derives the kernels' tables and uploads them to the
- * current device. */
+ * Validates the pattern, derives the kernels' row/column programs (LDS slot
+ * assignment of the working values, default LDS budget 40 KiB per 64-lane
+ * workgroup) and uploads them to the current device. */
 int wmpc_qdldl_plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out);
+/* Same derivation without a device: the plan only answers the inspection
+ * calls below (slots, programs); compute entry points reject it. */
+int wmpc_qdldl_plan_create_host(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out);
 int wmpc_qdldl_plan_destroy(wmpc_qdldl_plan* plan);
+/* Re-derives the programs for another LDS budget per workgroup (bytes, at most
+ * 160 KiB).  Live L entries that do not fit are re-read from global memory.
+ * No counterpart in the reference: the tuning knob of this backend. */
+int wmpc_qdldl_plan_set_lds_budget(wmpc_qdldl_plan* plan, int64_t bytes);
 /* Pattern sizes of a plan: n, nnz_a, nnz_l, max reach width. */
 int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a, int32_t* nnz_l,
                          int32_t* max_reach);
+/* Slot counts of a plan's programs, out[0..8] = reach slots, column-window
+ * slots, live-L slots in LDS, backward-window slots, live-L slots needed to keep
+ * every re-read on chip, L re-reads served from global memory, LDS bytes of the
+ * fused factor+solve, LDS bytes of the solve, LDS budget.  len >= 9. */
+int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len);
+/* Host copy of one program table (which: 0 row, 1 init, 2 proc, 3 upd, 4 bcol,
+ * 5 bent, 6 fcol, 7 bcol2, 8 fent, 9 ftpos); *len receives its length, out
+ * (capacity cap) its contents if non-NULL.  For inspection and CPU replay of
+ * the programs. */
+int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t* out, int64_t cap,
+                            int64_t* len);
 
 /* CompiledQDLDL.factor (warp_backend.py:1302-1349; kernel :104-173):
  * a_values [nnz_a][ld] (original upper-CSC order) -> lx [nnz_l][ld],
@@ -75,20 +94,23 @@ int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a
 int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, float* lx, float* d,
                           float* dinv, int64_t batch, int64_t ld, void* stream);
 
-/* CompiledQDLDL.solve (warp_backend.py:1351-1402; kernels :176-278):
- * x = Pᵀ L⁻ᵀ D⁻¹ L⁻¹ P rhs.  rhs, x [n][ld] in original order.
- * lx_ld / d_ld are the instance strides of the factor: ld for per-instance
- * factors, 0 for one factor shared by the whole batch.  work: [n][ld] scratch. */
+/* CompiledQDLDL.solve (warp_backend.py:1351-1402; kernels :176-454):
+ * x = Pᵀ L⁻ᵀ D⁻¹ L⁻¹ P rhs.  rhs, x [n][ld] in original order; x must not
+ * alias rhs (its rows stage the right-hand side).
+ * lx_ld / d_ld are the instance strides of the factor: >= batch for
+ * per-instance factors, 0 for one factor ([nnz_l], [n]) shared by the batch.
+ * work: scratch of work_len >= n*ld floats. */
 int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t lx_ld,
                          const float* dinv, int64_t d_ld, const float* rhs, float* x, float* work,
-                         int64_t batch, int64_t ld, void* stream);
+                         int64_t work_len, int64_t batch, int64_t ld, void* stream);
 
 /* CompiledQDLDL.factor_and_solve (warp_backend.py:1404-1408), one launch:
  * factor with the forward substitution fused into each row, then the backward
- * sweep.  Outputs x, lx, d, dinv; work: [n][ld] scratch. */
+ * sweep.  Outputs x, lx, d, dinv; work: scratch of work_len >= n*ld floats. */
 int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_values,
                                 const float* rhs, float* x, float* lx, float* d, float* dinv,
-                                float* work, int64_t batch, int64_t ld, void* stream);
+                                float* work, int64_t work_len, int64_t batch, int64_t ld,
+                                void* stream);
 
 /*
  * Fixed-pattern OSQP (warpmpc/jax_osqp/solver.py:685-982).  The descriptor
@@ -137,6 +159,13 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
                          const float* a_values, int64_t pm_ld, float* lx, float* dinv,
                          int64_t batch, int64_t ld, void* stream);
 
+/* Floats of scratch wmpc_osqp_solve_f32 needs, for factor_mode 0 (a factor is
+ * passed in), 1 (one shared matrix, pm_ld == 0, factored by the call) or 2
+ * (per-instance matrices factored by the call): 2*n_kkt*ld for the sweep's
+ * stash and staged right-hand side, plus (nnz_l + n_kkt) * (1 | ld) for the
+ * factor.  -1 on bad input. */
+int64_t wmpc_osqp_work_floats(const wmpc_osqp_plan* plan, int32_t factor_mode, int64_t ld);
+
 /*
  * CompiledOSQP.solve / solve_warm_start / solve_with_factor[_warm_start]
  * (solver.py:822-982) fused into one persistent launch: per instance the KKT
@@ -146,19 +175,17 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
  * p_values/a_values: [nnz][pm_ld], pm_ld = 0 when shared by the batch.
  * q [n][ld], l, u [m][ld]; x0 [n][ld], z0, y0 [m][ld] may be NULL (cold start).
  * lx/dinv: if non-NULL, a precomputed factor with instance stride f_ld
- * (0 = shared) and no factorization is done; otherwise the factor is computed
- * into the caller's lx_work/dinv_work ([nnz_l][ld], [n_kkt][ld]) — for a shared
- * matrix (pm_ld = 0) only instance 0 of the work buffers is used.
+ * (0 = shared, [nnz_l], [n_kkt]) and no factorization is done.
+ * work: scratch of work_len >= wmpc_osqp_work_floats(plan, mode, ld) floats.
  * Outputs: x [n][ld], z, y [m][ld], prim, dual, obj [ld].
- * work: [n_kkt][ld] scratch.
+ * The box projection propagates NaN like jnp.minimum/jnp.maximum.
  */
 int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         const float* a_values, int64_t pm_ld, const float* q, const float* l,
                         const float* u, const float* x0, const float* z0, const float* y0,
-                        const float* lx, const float* dinv, int64_t f_ld, float* lx_work,
-                        float* dinv_work, float* work, float* x, float* z, float* y, float* prim,
-                        float* dual, float* obj, int32_t max_iter, int64_t batch, int64_t ld,
-                        void* stream);
+                        const float* lx, const float* dinv, int64_t f_ld, float* work,
+                        int64_t work_len, float* x, float* z, float* y, float* prim, float* dual,
+                        float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream);
 
 #ifdef __cplusplus
 }

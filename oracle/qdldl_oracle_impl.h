@@ -233,9 +233,11 @@ int SFX(oracle_osqp_solve)(const oracle_osqp_desc* ds, int64_t batch, const REAL
           const REAL rinv = (REAL)ds->rho_inv_vec[i];
           const REAL ztilde = rhs[n + i] + rinv * sol[n + i];
           const REAL zr = alpha * ztilde + ((REAL)1 - alpha) * zs[i];
+          /* jnp.minimum(jnp.maximum(v, l), u) (solver.py:818): NaN in any
+             operand propagates */
           REAL zn = zr + rinv * ys[i];
-          zn = zn > ls[i] ? zn : ls[i];
-          zn = zn < us[i] ? zn : us[i];
+          zn = (zn != zn) ? zn : ((ls[i] != ls[i]) ? ls[i] : (zn > ls[i] ? zn : ls[i]));
+          zn = (zn != zn) ? zn : ((us[i] != us[i]) ? us[i] : (zn < us[i] ? zn : us[i]));
           ys[i] = ys[i] + (REAL)ds->rho_vec[i] * (zr - zn);
           zs[i] = zn;
         }
@@ -257,13 +259,14 @@ int SFX(oracle_osqp_solve)(const oracle_osqp_desc* ds, int64_t batch, const REAL
       for (int32_t j = 0; j < ds->nnz_p_full; ++j)
         px[ds->p_full_rows[j]] += pv[ds->p_full_value_map[j]] * xo[ds->p_full_cols[j]];
       REAL pr = 0, du = 0, quad = 0, lin = 0;
+      /* jnp.max(jnp.abs(.)) (solver.py:866-870): NaN propagates */
       for (int64_t i = 0; i < m; ++i) {
         const REAL r = fabs((double)(ax[i] - zo[i]));
-        pr = r > pr ? r : pr;
+        pr = (r != r || pr != pr) ? (REAL)NAN : (r > pr ? r : pr);
       }
       for (int64_t i = 0; i < n; ++i) {
         const REAL r = fabs((double)(px[i] + qb[i] + aty[i]));
-        du = r > du ? r : du;
+        du = (r != r || du != du) ? (REAL)NAN : (r > du ? r : du);
         quad += xo[i] * px[i];
         lin += qb[i] * xo[i];
       }

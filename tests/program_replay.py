@@ -1,0 +1,226 @@
+"""CPU replay of the HIP kernels' row/column programs (test helper, no GPU).
+
+The HIP library derives, per pattern, programs whose operands are LDS slots
+(reach slots Y, column window CW/FW, live-L slots AL, backward window BW) or L
+entries re-read from global memory.  This module replays exactly those
+programs for one instance in float64 with one Python dict per LDS region, so a
+slot that is reused while still live, or read before it is written, changes
+the result and fails the comparison with the oracle.  It mirrors
+``factor_rows``, ``forward_rows`` and ``backward_cols`` of
+``warpmpc_amd/csrc/wmpc_qdldl_device.h``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from warpmpc_amd import _lib
+from warpmpc_amd.qdldl.tables import ragged_program
+
+
+class HostPlan:
+    """A host-only ``wmpc_qdldl_plan`` and its program tables."""
+
+    def __init__(self, plan, lds_budget: int | None = None):
+        lib = _lib.load()
+        self.plan = plan
+        self.tables = ragged_program(plan)
+        pat = _lib.QdldlPattern()
+        pat.n, pat.nnz_a, pat.nnz_l = plan.n, plan.nnz_a, plan.nnz_l
+        for name, arr in self.tables.items():
+            setattr(pat, name, _lib.i32p(arr))
+        self._pat = pat
+        h = ctypes.c_void_p()
+        _lib.check(lib.wmpc_qdldl_plan_create_host(ctypes.byref(pat), ctypes.byref(h)), "plan_create_host")
+        self.handle = h
+        if lds_budget is not None:
+            _lib.check(lib.wmpc_qdldl_plan_set_lds_budget(h, int(lds_budget)), "set_lds_budget")
+        self.slots = _lib.plan_slots(h)
+        self.row = _lib.plan_program(h, "row").reshape(-1, 4)
+        self.init = _lib.plan_program(h, "init").reshape(-1, 2)
+        self.proc = _lib.plan_program(h, "proc").reshape(-1, 4)
+        self.upd = _lib.plan_program(h, "upd").reshape(-1, 2)
+        self.bcol = _lib.plan_program(h, "bcol").reshape(-1, 2)
+        self.bent = _lib.plan_program(h, "bent")
+        self.fcol = _lib.plan_program(h, "fcol").reshape(-1, 4)
+        self.bcol2 = _lib.plan_program(h, "bcol2").reshape(-1, 4)
+        self.fent = _lib.plan_program(h, "fent")
+        self.ftpos = _lib.plan_program(h, "ftpos")
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            _lib.load().wmpc_qdldl_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class _Region:
+    """One LDS region: slot -> value, with a bound check and a write check."""
+
+    def __init__(self, name: str, size: int):
+        self.name, self.size, self.v = name, size, {}
+
+    def __getitem__(self, s):
+        if not 0 <= s < self.size:
+            raise IndexError(f"{self.name} slot {s} outside [0, {self.size})")
+        if s not in self.v:
+            raise KeyError(f"{self.name} slot {s} read before any write")
+        return self.v[s]
+
+    def __setitem__(self, s, val):
+        if not 0 <= s < self.size:
+            raise IndexError(f"{self.name} slot {s} outside [0, {self.size})")
+        self.v[s] = val
+
+
+FIRST_TOUCH = 1 << 16
+
+
+def replay_column_sweeps(hp: HostPlan, lx, dinv, rhs_orig):
+    """``fwd_cols`` + ``bwd_cols`` of ``wmpc_stream.h`` for one instance.
+
+    The right-hand side is staged in first-touch order (``R[ftpos[c]] =
+    rhs[p[c]]``), the forward sweep scatters in CSC order and stashes the
+    D⁻¹-scaled result, the backward sweep gathers in descending CSC order.
+    """
+    n, nl = hp.plan.n, hp.plan.nnz_l
+    p = hp.tables["p"]
+    R = np.zeros(n)
+    R[hp.ftpos] = rhs_orig[p]
+    W = _Region("W", hp.slots["backwin"])
+    stash = np.zeros(n)
+    q = ir = 0
+    for c in range(n):
+        ws, qend = int(hp.fcol[c][0]), int(hp.fcol[c][1])
+        if ws >= 0:
+            val = W[ws]
+        else:
+            val = R[ir]
+            ir += 1
+        stash[c] = val * dinv[c]
+        while q < qend:
+            e = int(hp.fent[q])
+            slot = e & (FIRST_TOUCH - 1)
+            if e & FIRST_TOUCH:
+                W[slot] = R[ir] - lx[q] * val
+                ir += 1
+            else:
+                W[slot] = W[slot] - lx[q] * val
+            q += 1
+    assert ir == n, "first-touch stream not fully consumed"
+    x = np.zeros(n)
+    W = _Region("W", hp.slots["backwin"])
+    i = 0
+    for k in range(n):
+        ws, qbeg, pc = (int(v) for v in hp.bcol2[k][:3])
+        corr = 0.0
+        while nl - 1 - i >= qbeg:
+            qq = nl - 1 - i
+            corr += lx[qq] * W[int(hp.fent[qq]) & (FIRST_TOUCH - 1)]
+            i += 1
+        w = stash[n - 1 - k] - corr
+        if ws >= 0:
+            W[ws] = w
+        x[pc] = w
+    return x
+
+
+def replay_factor_solve(hp: HostPlan, a: np.ndarray, rhs: np.ndarray):
+    """factor_rows<FWD=true> + backward_cols for one instance.
+
+    a: [nnz_a] values in original upper-CSC order; rhs: [n] in original order.
+    Returns (x, lx, d, dinv) in float64.
+    """
+    n, nl = hp.plan.n, hp.plan.nnz_l
+    p = hp.tables["p"]
+    s = hp.slots
+    Y = _Region("Y", s["reach"])
+    CW = _Region("CW", s["colwin"])
+    FW = _Region("FW", s["colwin"])
+    AL = _Region("AL", s["live_l"])
+    lx = np.full(nl, np.nan)
+    d = np.zeros(n)
+    dinv = np.zeros(n)
+    stash = np.zeros(n)
+    r = j = u = 0
+    for k in range(n):
+        r1, i1, ck, dsrc = (int(v) for v in hp.row[k])
+        for t in range(r1 - r):
+            Y[t] = 0.0
+        while j < i1:
+            ys, src = hp.init[j]
+            Y[int(ys)] = Y[int(ys)] + a[src]
+            j += 1
+        dk = a[dsrc]
+        fw = rhs[p[k]]
+        t = 0
+        while r < r1:
+            cs, lidx, keep, uend = (int(v) for v in hp.proc[r])
+            val = Y[t]
+            while u < uend:
+                ys, ls = (int(v) for v in hp.upd[u])
+                lv = AL[ls] if ls >= 0 else lx[-ls - 1]
+                if np.isnan(lv):
+                    raise AssertionError(f"update {u} reads L slot {-ls - 1} before it is written")
+                Y[ys] = Y[ys] - lv * val
+                u += 1
+            lval = val * CW[cs]
+            lx[lidx] = lval
+            if keep >= 0:
+                AL[keep] = lval
+            dk -= val * lval
+            fw -= lval * FW[cs]
+            r += 1
+            t += 1
+        d[k] = dk
+        dinv[k] = 1.0 / dk
+        if ck >= 0:
+            CW[ck] = dinv[k]
+            FW[ck] = fw
+        stash[k] = fw
+    x = replay_backward(hp, lx, dinv, stash)
+    return x, lx, d, dinv
+
+
+def replay_backward(hp: HostPlan, lx, dinv, stash):
+    n = hp.plan.n
+    p = hp.tables["p"]
+    BW = _Region("BW", hp.slots["backwin"])
+    x = np.zeros(n)
+    for c in range(n - 1, -1, -1):
+        bs, qend = (int(v) for v in hp.bcol[c])
+        q = int(hp.bcol[c - 1][1]) if c > 0 else 0
+        acc = stash[c] * dinv[c]
+        for qq in range(q, qend):
+            acc -= lx[qq] * BW[int(hp.bent[qq])]
+        if bs >= 0:
+            BW[bs] = acc
+        x[p[c]] = acc
+    return x
+
+
+def replay_solve(hp: HostPlan, lx, dinv, rhs):
+    """forward_rows + backward_cols for one instance with a given factor."""
+    n = hp.plan.n
+    p = hp.tables["p"]
+    FW = _Region("FW", hp.slots["colwin"])
+    stash = np.zeros(n)
+    r = 0
+    for k in range(n):
+        r1, _, ck, _ = (int(v) for v in hp.row[k])
+        acc = rhs[p[k]]
+        while r < r1:
+            cs, lidx = int(hp.proc[r][0]), int(hp.proc[r][1])
+            acc -= lx[lidx] * FW[cs]
+            r += 1
+        if ck >= 0:
+            FW[ck] = acc
+        stash[k] = acc
+    return replay_backward(hp, lx, dinv, stash)
+
+
+__all__ = ["HostPlan", "replay_backward", "replay_column_sweeps", "replay_factor_solve", "replay_solve"]

@@ -92,18 +92,26 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_last_error": (ctypes.c_char_p, []),
     "wmpc_version": (ctypes.c_char_p, []),
     "wmpc_qdldl_plan_create": (ctypes.c_int, [ctypes.POINTER(QdldlPattern), ctypes.POINTER(_vp)]),
+    "wmpc_qdldl_plan_create_host": (ctypes.c_int, [ctypes.POINTER(QdldlPattern), ctypes.POINTER(_vp)]),
     "wmpc_qdldl_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_qdldl_plan_set_lds_budget": (ctypes.c_int, [_vp, _i64]),
     "wmpc_qdldl_plan_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
+    "wmpc_qdldl_plan_slots": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), _i32]),
+    "wmpc_qdldl_plan_program": (ctypes.c_int, [_vp, _i32, _i32p, _i64, ctypes.POINTER(ctypes.c_int64)]),
     "wmpc_qdldl_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
-    "wmpc_qdldl_solve_f32": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _vp]),
-    "wmpc_qdldl_factor_solve_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_qdldl_solve_f32": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "wmpc_qdldl_factor_solve_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64,
+                                                   _vp]),
     "wmpc_osqp_plan_create": (ctypes.c_int, [ctypes.POINTER(OsqpDesc), _vp, ctypes.POINTER(_vp)]),
     "wmpc_osqp_plan_destroy": (ctypes.c_int, [_vp]),
     "wmpc_osqp_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp]),
+    "wmpc_osqp_work_floats": (ctypes.c_int64, [_vp, _i32, _i64]),
     "wmpc_osqp_solve_f32": (ctypes.c_int, [
-        _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+        _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
 }
+
+PROGRAM_TABLES = ("row", "init", "proc", "upd", "bcol", "bent", "fcol", "bcol2", "fent", "ftpos")
 
 
 class HipError(RuntimeError):
@@ -136,6 +144,27 @@ def check(rc: int, what: str) -> None:
         raise HipError(f"{what} failed (code {rc}): {msg}")
 
 
+def plan_slots(handle) -> dict[str, int]:
+    """LDS slot counts of a QDLDL plan (``wmpc_qdldl_plan_slots``)."""
+    out = (ctypes.c_int64 * 9)()
+    check(load().wmpc_qdldl_plan_slots(handle, out, 9), "wmpc_qdldl_plan_slots")
+    keys = ("reach", "colwin", "live_l", "backwin", "live_l_needed", "global_rereads", "lds_factor_solve",
+            "lds_solve", "lds_budget")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def plan_program(handle, name: str) -> np.ndarray:
+    """Host copy of one program table of a QDLDL plan (``wmpc_qdldl_plan_program``)."""
+    which = PROGRAM_TABLES.index(name)
+    n = ctypes.c_int64()
+    lib = load()
+    check(lib.wmpc_qdldl_plan_program(handle, which, None, 0, ctypes.byref(n)), "wmpc_qdldl_plan_program")
+    out = np.zeros(max(int(n.value), 1), dtype=np.int32)
+    check(lib.wmpc_qdldl_plan_program(handle, which, out.ctypes.data_as(_i32p), out.size, ctypes.byref(n)),
+          "wmpc_qdldl_plan_program")
+    return out[: int(n.value)]
+
+
 def i32p(a: np.ndarray):
     return a.ctypes.data_as(_i32p)
 
@@ -144,4 +173,5 @@ def f64p(a: np.ndarray):
     return a.ctypes.data_as(_f64p)
 
 
-__all__ = ["HipError", "LIB_PATH", "OsqpDesc", "QdldlPattern", "SIGNATURES", "check", "load"]
+__all__ = ["HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
+           "load", "plan_program", "plan_slots"]

@@ -33,34 +33,19 @@ void clear_error();
     }                                       \
   } while (0)
 
-// Device-resident program tables of one QDLDL pattern.  All int32, one
-// allocation (`block`).  Row k of the factorization:
-//   reach slots s = 0..R-1 correspond to proc entries r = proc_ptr[k] + s;
-//   init entry j adds a[init_src[j]] into slot init_slot[j];
-//   proc entry r (column c = proc_col[r]) first applies the updates
-//   y[upd_slot[u]] -= L[lp[c] + (u - upd_ptr[r])] * y[s] for u in
-//   [upd_ptr[r], upd_ptr[r+1]), then writes L[k, c] to slot proc_lidx[r].
-struct QdldlTables {
-  int32_t n = 0, nnz_a = 0, nnz_l = 0, max_reach = 0, n_proc = 0, n_upd = 0;
-  const int32_t* diag_src = nullptr;  // [n] original value slot of A_kk
-  const int32_t* init_ptr = nullptr;  // [n+1]
-  const int32_t* init_slot = nullptr;
-  const int32_t* init_src = nullptr;  // original value slot
-  const int32_t* proc_ptr = nullptr;  // [n+1]
-  const int32_t* proc_col = nullptr;
-  const int32_t* proc_lidx = nullptr;
-  const int32_t* upd_ptr = nullptr;   // [n_proc+1]
-  const int32_t* upd_slot = nullptr;
-  const int32_t* lp = nullptr;        // [n+1]
-  const int32_t* li = nullptr;        // [nnz_l]
-  const int32_t* p = nullptr;         // [n]
-  const int32_t* pinv = nullptr;      // [n]
-};
+inline int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string(what) + " launch: " + hipGetErrorString(e));
+    return WMPC_ERR_HIP;
+  }
+  return WMPC_OK;
+}
+
+// Largest dynamic LDS a workgroup may request on gfx950 (160 KiB per CU).
+constexpr size_t kMaxLdsBytes = 160 * 1024;
+// Default LDS budget of one 64-lane workgroup: four workgroups (one wavefront
+// per SIMD) share a CU.
+constexpr size_t kDefaultLdsBytes = 40 * 1024;
 
 }  // namespace wmpc
-
-struct wmpc_qdldl_plan {
-  wmpc::QdldlTables t;
-  void* block = nullptr;
-  int device = 0;
-};

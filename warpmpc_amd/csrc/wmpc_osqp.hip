@@ -1,43 +1,53 @@
 // Fused fixed-iteration OSQP ADMM for MI355X (gfx950) behind include/warpmpc_hip.h.
 //
 // One launch runs, per instance (one lane each): the KKT value assembly and
-// LDLᵀ factorization (solver.py:764-776, 799-804), `max_iter` iterations of
-// the ADMM step of solver.py:806-820 (rhs build, quasi-definite solve,
-// relaxation, box projection, dual update), the unscaling and the residual /
-// objective evaluation of solver.py:863-872.  The iterate lives in the caller's
-// x / z / y output buffers (batch-major), the solve in the work buffer.
+// LDLᵀ factorization (solver.py:764-776, 799-804) unless a factor is given,
+// `max_iter` iterations of the ADMM step of solver.py:806-820, the unscaling and
+// the residual / objective evaluation of solver.py:863-872.
+//
+// Per iteration: the forward and backward substitutions (prefetched column
+// sweeps of wmpc_stream.h) solve the KKT system into the stash; one pass over
+// the entries (variables, then constraints: no branches, independent
+// iterations) applies the relaxation, box projection and dual update and
+// writes the next right-hand side [sigma x - q; z - rho⁻¹ y] in the sweep's
+// first-touch order.  Scratch: stash and right-hand side, [n_kkt][ld] each.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
-#include "wmpc_qdldl_device.h"
+#include "wmpc_plan.h"
+#include "wmpc_stream.h"
+
 
 namespace wmpc {
 
-constexpr int kOsqpBlock = 256;
-
 struct OsqpDev {
-  int32_t n = 0, m = 0, nnz_p = 0, nnz_a = 0, nnz_kkt = 0, nnz_p_full = 0;
+  int32_t n = 0, m = 0, nnz_p = 0, nnz_a = 0, nnz_kkt = 0;
   float sigma = 0.f, alpha = 0.f;
   // KKT value sources, indexed by original KKT slot
   const int32_t* src_kind = nullptr;
   const int32_t* src_idx = nullptr;
   const float* src_scale = nullptr;
   const float* src_off = nullptr;
-  const float* q_scale = nullptr;
-  const float* sd = nullptr;
-  const float* sdinv = nullptr;
-  const float* se = nullptr;
-  const float* seinv = nullptr;
-  const float* y_scale = nullptr;
-  const float* y_unscale = nullptr;
-  const float* rho = nullptr;
-  const float* rho_inv = nullptr;
-  const int32_t* a_rows = nullptr;
-  const int32_t* a_cols = nullptr;
-  const int32_t* pf_rows = nullptr;
-  const int32_t* pf_cols = nullptr;
-  const int32_t* pf_map = nullptr;
+  // update-pass entries: variables {c, o, first-touch position, q scale},
+  // constraints {c, i, first-touch position, rho⁻¹} + {rho, E, 0, 0}
+  int32_t nvar = 0, ncon = 0;
+  const int4* vtab = nullptr;
+  const int4* ctab = nullptr;
+  const float* sd = nullptr;      // D      [n]
+  const float* sdinv = nullptr;   // D⁻¹    [n]
+  const float* se = nullptr;      // E      [m]
+  const float* seinv = nullptr;   // E⁻¹    [m]
+  const float* y_scale = nullptr;    // [m]
+  const float* y_unscale = nullptr;  // [m]
+  // gather tables of the residuals: rows of A, columns of A, rows of full P
+  const int32_t* arow_ptr = nullptr;  // [m+1]
+  const int2* arow = nullptr;         // {col, A value slot}
+  const int32_t* acol_ptr = nullptr;  // [n+1]
+  const int2* acol = nullptr;         // {row, A value slot}
+  const int32_t* prow_ptr = nullptr;  // [n+1]
+  const int2* prow = nullptr;         // {col, P value slot}
 };
 
 }  // namespace wmpc
@@ -50,112 +60,136 @@ struct wmpc_osqp_plan {
 
 namespace wmpc {
 
-// rhs of the KKT solve in original order, built from the iterate on the fly:
-// [sigma x - q_scaled; z - rho⁻¹ y]  (solver.py:808-810)
-struct AdmmRhs {
-  const OsqpDev* dv;
-  // not __restrict__: these alias the iterate the kernel updates between solves
-  const float* x;
-  const float* z;
-  const float* y;
-  const float* q;
-  int64_t ld;
-  float sigma;
-  int32_t n;
-  __device__ __forceinline__ float operator()(int32_t o, int64_t b) const {
-    if (o < n) return sigma * x[(int64_t)o * ld + b] - q[(int64_t)o * ld + b] * dv->q_scale[o];
-    const int32_t i = o - n;
-    return z[(int64_t)i * ld + b] - dv->rho_inv[i] * y[(int64_t)i * ld + b];
+// Solution entry c of the backward sweep goes back into the stash.
+struct StashOut {
+  float* __restrict__ stash;
+  int64_t ld, b;
+  __device__ __forceinline__ void operator()(int32_t c, int32_t, float v) const {
+    (stash + (int64_t)c * ld)[(uint32_t)b] = v;
   }
 };
 
+// One pass over the iterate (solver.py:806-820): unless FIRST, the relaxation,
+// projection and dual update with the KKT solution S; unless LAST, the next
+// right-hand side R (solver.py:808-810) at the entry's first-touch position.
+template <bool FIRST, bool LAST>
+__device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, const Strided& R,
+                                          const Strided& X, const Strided& Z, const Strided& Y,
+                                          const CStrided& Q, const CStrided& Lo,
+                                          const CStrided& Up) {
+  const float alpha = dv.alpha, oma = 1.0f - dv.alpha, sigma = dv.sigma;
+#pragma unroll 4
+  for (int32_t v = 0; v < dv.nvar; ++v) {
+    const int4 t = dv.vtab[v];
+    const float xo = X(t.y);
+    float xn = xo;
+    if constexpr (!FIRST) {
+      xn = alpha * S(t.x) + oma * xo;
+      X(t.y) = xn;
+    }
+    if constexpr (!LAST) R(t.z) = sigma * xn - __int_as_float(t.w) * Q(t.y);
+  }
+#pragma unroll 4
+  for (int32_t k = 0; k < dv.ncon; ++k) {
+    const int4 t = dv.ctab[2 * k];
+    const float rinv = __int_as_float(t.w);
+    const float zo = Z(t.y), yo = Y(t.y);
+    float zn = zo, yn = yo;
+    if constexpr (!FIRST) {
+      const int4 t2 = dv.ctab[2 * k + 1];
+      const float rho = __int_as_float(t2.x), e = __int_as_float(t2.y);
+      const float rz = zo - rinv * yo;
+      const float zt = rz + rinv * S(t.x);
+      const float zr = alpha * zt + oma * zo;
+      zn = nan_min(nan_max(zr + rinv * yo, Lo(t.y) * e), Up(t.y) * e);
+      yn = yo + rho * (zr - zn);
+      Z(t.y) = zn;
+      Y(t.y) = yn;
+    }
+    if constexpr (!LAST) R(t.z) = zn - rinv * yn;
+  }
+}
+
+// SHARED: one factor for the batch ([nnz_l], [n_kkt]); FACTOR: assemble and
+// factor each instance's KKT into the work area first.
 template <bool SHARED, bool FACTOR>
-__global__ __launch_bounds__(kOsqpBlock) void k_osqp_admm(
-    QdldlTables t, OsqpDev dv, const float* __restrict__ pv, const float* __restrict__ av,
+__global__ __launch_bounds__(kWave) void k_osqp_admm(
+    QdldlProg P, OsqpDev dv, const float* __restrict__ pv, const float* __restrict__ av,
     int64_t pm_ld, const float* __restrict__ q, const float* __restrict__ l,
     const float* __restrict__ u, const float* __restrict__ x0, const float* __restrict__ z0,
-    const float* __restrict__ y0, float* __restrict__ lx, float* __restrict__ dinv,
-    int64_t f_ld, float* __restrict__ w, float* __restrict__ x, float* __restrict__ z,
+    const float* __restrict__ y0, float* __restrict__ lx, float* __restrict__ dinv, int64_t f_ld,
+    float* __restrict__ w, float* __restrict__ rbuf, float* __restrict__ x, float* __restrict__ z,
     float* __restrict__ y, float* __restrict__ prim, float* __restrict__ dual,
-    float* __restrict__ obj, int32_t max_iter, int64_t batch, int64_t ld) {
-  extern __shared__ float ys[];
-  const int64_t b = (int64_t)blockIdx.x * kOsqpBlock + threadIdx.x;
-  if (b >= batch) return;
+    float* __restrict__ obj, int32_t max_iter, int64_t batch, int64_t ld, int32_t lane_slots) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
+  // lanes past the batch mirror the last instance (identical loads and stores)
+  // so that every lane takes part in the wave-shared staging of the sweeps
+  const int64_t b = b0 < batch ? b0 : batch - 1;
+  float* mylds = lds + lane;
+  char* wave_lds = reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave);
   const int64_t pb = pm_ld ? b : 0;
-  const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A (shared: one contiguous column)
+  const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
   const int32_t n = dv.n, m = dv.m;
+  const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
   if constexpr (FACTOR) {
-    // per-instance KKT assembly + factorization; d goes to the (later reused) work rows
     KktSource src{pv, av, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-    factor_lane<KktSource, false>(t, src, pb, b, ld, lx, nullptr, dinv, ys + threadIdx.x,
-                                  kOsqpBlock, ArrayRhs{nullptr, 0}, nullptr);
+    factor_rows<false>(P, src, pb, Strided{lx, fs, fb}, Strided{nullptr, 0, 0},
+                       Strided{dinv, fs, fb}, mylds, NoRhs{}, Strided{nullptr, 0, 0});
   }
-  const float* __restrict__ lxr = lx;
-  const float* __restrict__ dinvr = dinv;
+  const Strided X{x, ld, b}, Z{z, ld, b}, Y{y, ld, b}, R{rbuf, ld, b};
+  const CStrided Q{q, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, S{w, ld, b};
   // scaled initial iterate (solver.py:842-847)
-  for (int32_t i = 0; i < n; ++i)
-    x[(int64_t)i * ld + b] = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
+  for (int32_t i = 0; i < n; ++i) X(i) = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
   for (int32_t i = 0; i < m; ++i) {
-    z[(int64_t)i * ld + b] = z0 ? z0[(int64_t)i * ld + b] * dv.se[i] : 0.0f;
-    y[(int64_t)i * ld + b] = y0 ? y0[(int64_t)i * ld + b] * dv.y_scale[i] : 0.0f;
+    Z(i) = z0 ? z0[(int64_t)i * ld + b] * dv.se[i] : 0.0f;
+    Y(i) = y0 ? y0[(int64_t)i * ld + b] * dv.y_scale[i] : 0.0f;
   }
-  const float alpha = dv.alpha, oma = 1.0f - dv.alpha;
-  const AdmmRhs rhs{&dv, x, z, y, q, ld, dv.sigma, n};
-  for (int32_t it = 0; it < max_iter; ++it) {
-    forward_rows<SHARED>(t, lxr, f_ld, b, ld, rhs, w);
-    backward_cols<SHARED>(t, lxr, dinvr, f_ld, b, ld, w);
-    for (int32_t i = 0; i < n; ++i) {
-      const int64_t xi = (int64_t)i * ld + b;
-      const float xt = w[(int64_t)t.pinv[i] * ld + b];
-      x[xi] = alpha * xt + oma * x[xi];
-    }
-    for (int32_t i = 0; i < m; ++i) {
-      const int64_t zi = (int64_t)i * ld + b;
-      const float rinv = dv.rho_inv[i];
-      const float zo = z[zi], yo = y[zi];
-      const float rz = zo - rinv * yo;
-      const float zt = rz + rinv * w[(int64_t)t.pinv[n + i] * ld + b];
-      const float zr = alpha * zt + oma * zo;
-      const float lo = l[zi] * dv.se[i], hi = u[zi] * dv.se[i];
-      const float zn = fminf(fmaxf(zr + rinv * yo, lo), hi);
-      y[zi] = yo + dv.rho[i] * (zr - zn);
-      z[zi] = zn;
+  if (max_iter > 0) {
+    const SweepProg SP = sweep_prog(P);
+    admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+    for (int32_t it = 0; it < max_iter; ++it) {
+      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, mylds, wave_lds, lane);
+      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, mylds, wave_lds, lane, StashOut{w, ld, b});
+      if (it + 1 < max_iter)
+        admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+      else
+        admm_pass<false, true>(dv, S, R, X, Z, Y, Q, Lo, Up);
     }
   }
   // unscale (solver.py:863-865)
-  for (int32_t i = 0; i < n; ++i) x[(int64_t)i * ld + b] *= dv.sd[i];
+  for (int32_t i = 0; i < n; ++i) X(i) *= dv.sd[i];
   for (int32_t i = 0; i < m; ++i) {
-    z[(int64_t)i * ld + b] *= dv.seinv[i];
-    y[(int64_t)i * ld + b] *= dv.y_unscale[i];
+    Z(i) *= dv.seinv[i];
+    Y(i) *= dv.y_unscale[i];
   }
-  // residuals and objective with the unscaled P, A, q (solver.py:866-872)
-  float* ax = w;                       // rows [0, m)
-  float* px = w + (int64_t)m * ld;     // rows [m, m+n)
-  float* aty = w + (int64_t)(m + n) * ld;  // rows [m+n, m+2n)
-  for (int32_t i = 0; i < m; ++i) ax[(int64_t)i * ld + b] = 0.0f;
-  for (int32_t i = 0; i < n; ++i) {
-    px[(int64_t)i * ld + b] = 0.0f;
-    aty[(int64_t)i * ld + b] = 0.0f;
+  // residuals and objective with the unscaled P, A, q (solver.py:866-872), in
+  // gather form: no scatter scratch
+  float pr = 0.0f;
+  for (int32_t i = 0; i < m; ++i) {
+    float s = 0.0f;
+    for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
+      const int2 t = dv.arow[e];
+      s += av[(int64_t)t.y * pes + pb] * X(t.x);
+    }
+    pr = nan_max(pr, fabsf(s - Z(i)));
   }
-  for (int32_t j = 0; j < dv.nnz_a; ++j) {
-    const float a = av[(int64_t)j * pes + pb];
-    const int32_t r = dv.a_rows[j], c = dv.a_cols[j];
-    ax[(int64_t)r * ld + b] += a * x[(int64_t)c * ld + b];
-    aty[(int64_t)c * ld + b] += a * y[(int64_t)r * ld + b];
-  }
-  for (int32_t j = 0; j < dv.nnz_p_full; ++j) {
-    const float pval = pv[(int64_t)dv.pf_map[j] * pes + pb];
-    px[(int64_t)dv.pf_rows[j] * ld + b] += pval * x[(int64_t)dv.pf_cols[j] * ld + b];
-  }
-  float pr = 0.0f, du = 0.0f, quad = 0.0f, lin = 0.0f;
-  for (int32_t i = 0; i < m; ++i)
-    pr = fmaxf(pr, fabsf(ax[(int64_t)i * ld + b] - z[(int64_t)i * ld + b]));
-  for (int32_t i = 0; i < n; ++i) {
-    const int64_t ii = (int64_t)i * ld + b;
-    const float xi = x[ii], pxi = px[ii], qi = q[ii];
-    du = fmaxf(du, fabsf(pxi + qi + aty[ii]));
-    quad += xi * pxi;
-    lin += qi * xi;
+  float du = 0.0f, quad = 0.0f, lin = 0.0f;
+  for (int32_t j = 0; j < n; ++j) {
+    float px = 0.0f, aty = 0.0f;
+    for (int32_t e = dv.prow_ptr[j]; e < dv.prow_ptr[j + 1]; ++e) {
+      const int2 t = dv.prow[e];
+      px += pv[(int64_t)t.y * pes + pb] * X(t.x);
+    }
+    for (int32_t e = dv.acol_ptr[j]; e < dv.acol_ptr[j + 1]; ++e) {
+      const int2 t = dv.acol[e];
+      aty += av[(int64_t)t.y * pes + pb] * Y(t.x);
+    }
+    const float xj = X(j), qj = Q(j);
+    du = nan_max(du, fabsf(px + qj + aty));
+    quad += xj * px;
+    lin += qj * xj;
   }
   prim[b] = pr;
   dual[b] = du;
@@ -163,26 +197,55 @@ __global__ __launch_bounds__(kOsqpBlock) void k_osqp_admm(
 }
 
 // KKT assembly + factorization only (CompiledOSQP.factor).
-__global__ __launch_bounds__(kOsqpBlock) void k_osqp_factor(QdldlTables t, KktSource src,
-                                                            float* __restrict__ lx,
-                                                            float* __restrict__ d,
-                                                            float* __restrict__ dinv,
-                                                            int64_t batch, int64_t ld) {
-  extern __shared__ float ys[];
-  const int64_t b = (int64_t)blockIdx.x * kOsqpBlock + threadIdx.x;
+__global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource src,
+                                                       float* __restrict__ lx,
+                                                       float* __restrict__ dinv, int64_t batch,
+                                                       int64_t ld) {
+  extern __shared__ float lds[];
+  const int64_t b = (int64_t)blockIdx.x * kWave + threadIdx.x;
   if (b >= batch) return;
   const int64_t pb = src.pm_ld ? b : 0;
-  factor_lane<KktSource, false>(t, src, pb, b, ld, lx, d, dinv, ys + threadIdx.x, kOsqpBlock,
-                                ArrayRhs{nullptr, 0}, nullptr);
+  factor_rows<false>(P, src, pb, Strided{lx, ld, b}, Strided{nullptr, 0, 0},
+                     Strided{dinv, ld, b}, lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
 }
 
-static int osqp_check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    set_error(std::string(what) + " launch: " + hipGetErrorString(e));
-    return WMPC_ERR_HIP;
+static size_t factor_lds(const QdldlProg& P) {
+  return (size_t)factor_lds_slots(P, false) * kWave * sizeof(float);
+}
+static int32_t admm_lane_slots(const QdldlProg& P, bool shared, bool factor) {
+  int32_t s = sweep_lane_slots(sweep_prog(P), shared);
+  if (factor) s = std::max(s, factor_lds_slots(P, false));
+  return s;
+}
+static size_t admm_lds(const QdldlProg& P, bool shared, bool factor) {
+  return (size_t)admm_lane_slots(P, shared, factor) * kWave * sizeof(float) + sweep_wave_bytes();
+}
+
+template <class K>
+static int osqp_prepare_lds(K kernel, size_t bytes) {
+  if (bytes > kMaxLdsBytes) {
+    set_error("pattern needs more LDS than a workgroup can have");
+    return WMPC_ERR_UNSUPPORTED;
   }
+  if (bytes > 64 * 1024)
+    WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
   return WMPC_OK;
+}
+
+// CSR-style gather table of (row, col, slot) triples grouped by `key`.
+static void group_by(int32_t groups, const std::vector<int32_t>& key, const std::vector<int32_t>& other,
+                     const std::vector<int32_t>& slot, std::vector<int32_t>& ptr, std::vector<int32_t>& ent) {
+  ptr.assign(groups + 1, 0);
+  for (int32_t k : key) ++ptr[k + 1];
+  for (int32_t g = 0; g < groups; ++g) ptr[g + 1] += ptr[g];
+  ent.assign(2 * key.size(), 0);
+  std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+  for (size_t j = 0; j < key.size(); ++j) {
+    const int32_t at = fill[key[j]]++;
+    ent[2 * (size_t)at] = other[j];
+    ent[2 * (size_t)at + 1] = slot[j];
+  }
 }
 
 }  // namespace wmpc
@@ -196,13 +259,14 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   clear_error();
   WMPC_REQUIRE(ds && kkt && out, "null argument");
   *out = nullptr;
+  WMPC_REQUIRE(kkt->block, "host-only KKT plan: create it with wmpc_qdldl_plan_create");
   const int32_t n = ds->n, m = ds->m;
-  WMPC_REQUIRE(n > 0 && m >= 0 && kkt->t.n == n + m, "KKT plan dimension must be n + m");
-  WMPC_REQUIRE(ds->nnz_kkt == kkt->t.nnz_a, "KKT nonzero count differs from the QDLDL plan");
+  WMPC_REQUIRE(n > 0 && m >= 0 && kkt->host.n == n + m, "KKT plan dimension must be n + m");
+  WMPC_REQUIRE(ds->nnz_kkt == kkt->host.nnz_a, "KKT nonzero count differs from the QDLDL plan");
   WMPC_REQUIRE(ds->kkt_base && ds->p_to_kkt && ds->p_diag_sigma && ds->a_to_kkt && ds->p_scale &&
                    ds->a_scale && ds->q_scale && ds->scaling_d && ds->scaling_dinv &&
                    ds->scaling_e && ds->scaling_einv && ds->y_scale && ds->y_unscale &&
-                   ds->rho_vec && ds->rho_inv_vec && ds->a_rows && ds->a_cols &&
+                   ds->rho_vec && ds->rho_inv_vec && (ds->nnz_a == 0 || (ds->a_rows && ds->a_cols)) &&
                    (ds->nnz_p_full == 0 || (ds->p_full_rows && ds->p_full_cols && ds->p_full_value_map)),
                "descriptor has a null array");
   std::vector<int32_t> kind(ds->nnz_kkt, 2), idx(ds->nnz_kkt, 0);
@@ -224,20 +288,53 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
     scale[s] = (float)ds->a_scale[j];
     off[s] = 0.f;
   }
-  for (int32_t j = 0; j < ds->nnz_a; ++j)
+  std::vector<int32_t> a_rows(ds->nnz_a), a_cols(ds->nnz_a), a_slot(ds->nnz_a);
+  for (int32_t j = 0; j < ds->nnz_a; ++j) {
     WMPC_REQUIRE(ds->a_rows[j] >= 0 && ds->a_rows[j] < m && ds->a_cols[j] >= 0 && ds->a_cols[j] < n,
                  "a_rows/a_cols out of range");
-  for (int32_t j = 0; j < ds->nnz_p_full; ++j)
+    a_rows[j] = ds->a_rows[j];
+    a_cols[j] = ds->a_cols[j];
+    a_slot[j] = j;
+  }
+  std::vector<int32_t> pf_rows(ds->nnz_p_full), pf_cols(ds->nnz_p_full), pf_map(ds->nnz_p_full);
+  for (int32_t j = 0; j < ds->nnz_p_full; ++j) {
     WMPC_REQUIRE(ds->p_full_rows[j] >= 0 && ds->p_full_rows[j] < n && ds->p_full_cols[j] >= 0 &&
                      ds->p_full_cols[j] < n && ds->p_full_value_map[j] >= 0 &&
                      ds->p_full_value_map[j] < ds->nnz_p,
                  "p_full arrays out of range");
-  // one host blob: ints then floats, every sub-array 256-byte aligned
+    pf_rows[j] = ds->p_full_rows[j];
+    pf_cols[j] = ds->p_full_cols[j];
+    pf_map[j] = ds->p_full_value_map[j];
+  }
+  std::vector<int32_t> arow_ptr, arow, acol_ptr, acol, prow_ptr, prow;
+  group_by(m, a_rows, a_cols, a_slot, arow_ptr, arow);
+  group_by(n, a_cols, a_rows, a_slot, acol_ptr, acol);
+  group_by(n, pf_rows, pf_cols, pf_map, prow_ptr, prow);
+  auto bits = [](double v) {
+    float f = (float)v;
+    int32_t i;
+    std::memcpy(&i, &f, 4);
+    return i;
+  };
+  std::vector<int32_t> vtab, ctab;
+  for (int32_t c = 0; c < n + m; ++c) {
+    const int32_t o = kkt->host.p[c];
+    const int32_t ft = kkt->host.ftpos[c];
+    if (o < n) {
+      vtab.insert(vtab.end(), {c, o, ft, bits(ds->q_scale[o])});
+    } else {
+      const int32_t i = o - n;
+      ctab.insert(ctab.end(), {c, i, ft, bits(ds->rho_inv_vec[i]), bits(ds->rho_vec[i]),
+                               bits(ds->scaling_e[i]), 0, 0});
+    }
+  }
+  // one host blob: every sub-array 256-byte aligned
   std::vector<uint32_t> blob;
   std::vector<size_t> offs;
   auto app_i = [&](const int32_t* s, size_t c) {
     offs.push_back(blob.size());
     for (size_t i = 0; i < c; ++i) blob.push_back((uint32_t)s[i]);
+    blob.push_back(0);
     while (blob.size() % 64) blob.push_back(0);
   };
   auto app_f = [&](const float* s, size_t c) {
@@ -247,29 +344,32 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
       std::memcpy(&v, &s[i], 4);
       blob.push_back(v);
     }
+    blob.push_back(0);
     while (blob.size() % 64) blob.push_back(0);
   };
-  auto conv = [](const double* s, size_t c) {
-    std::vector<float> o(c);
-    for (size_t i = 0; i < c; ++i) o[i] = (float)s[i];
-    return o;
+  auto app_d = [&](const double* s, size_t c) {
+    std::vector<float> v(c);
+    for (size_t i = 0; i < c; ++i) v[i] = (float)s[i];
+    app_f(v.data(), c);
   };
   app_i(kind.data(), kind.size());
   app_i(idx.data(), idx.size());
   app_f(scale.data(), scale.size());
   app_f(off.data(), off.size());
-  const double* vecs[] = {ds->q_scale, ds->scaling_d, ds->scaling_dinv, ds->scaling_e, ds->scaling_einv,
-                          ds->y_scale, ds->y_unscale, ds->rho_vec, ds->rho_inv_vec};
-  const int32_t lens[] = {n, n, n, m, m, m, m, m, m};
-  for (int i = 0; i < 9; ++i) {
-    auto v = conv(vecs[i], lens[i]);
-    app_f(v.data(), v.size());
-  }
-  app_i(ds->a_rows, ds->nnz_a);
-  app_i(ds->a_cols, ds->nnz_a);
-  app_i(ds->p_full_rows, ds->nnz_p_full);
-  app_i(ds->p_full_cols, ds->nnz_p_full);
-  app_i(ds->p_full_value_map, ds->nnz_p_full);
+  app_i(vtab.data(), vtab.size());
+  app_i(ctab.data(), ctab.size());
+  app_d(ds->scaling_d, n);
+  app_d(ds->scaling_dinv, n);
+  app_d(ds->scaling_e, m);
+  app_d(ds->scaling_einv, m);
+  app_d(ds->y_scale, m);
+  app_d(ds->y_unscale, m);
+  app_i(arow_ptr.data(), arow_ptr.size());
+  app_i(arow.data(), arow.size());
+  app_i(acol_ptr.data(), acol_ptr.size());
+  app_i(acol.data(), acol.size());
+  app_i(prow_ptr.data(), prow_ptr.size());
+  app_i(prow.data(), prow.size());
 
   auto* plan = new wmpc_osqp_plan();
   hipError_t e = hipMalloc(&plan->block, blob.size() * 4);
@@ -288,27 +388,28 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   d.nnz_p = ds->nnz_p;
   d.nnz_a = ds->nnz_a;
   d.nnz_kkt = ds->nnz_kkt;
-  d.nnz_p_full = ds->nnz_p_full;
   d.sigma = (float)ds->sigma;
   d.alpha = (float)ds->alpha;
   d.src_kind = (const int32_t*)(base + offs[i++]);
   d.src_idx = (const int32_t*)(base + offs[i++]);
   d.src_scale = (const float*)(base + offs[i++]);
   d.src_off = (const float*)(base + offs[i++]);
-  d.q_scale = (const float*)(base + offs[i++]);
+  d.nvar = n;
+  d.ncon = m;
+  d.vtab = (const int4*)(base + offs[i++]);
+  d.ctab = (const int4*)(base + offs[i++]);
   d.sd = (const float*)(base + offs[i++]);
   d.sdinv = (const float*)(base + offs[i++]);
   d.se = (const float*)(base + offs[i++]);
   d.seinv = (const float*)(base + offs[i++]);
   d.y_scale = (const float*)(base + offs[i++]);
   d.y_unscale = (const float*)(base + offs[i++]);
-  d.rho = (const float*)(base + offs[i++]);
-  d.rho_inv = (const float*)(base + offs[i++]);
-  d.a_rows = (const int32_t*)(base + offs[i++]);
-  d.a_cols = (const int32_t*)(base + offs[i++]);
-  d.pf_rows = (const int32_t*)(base + offs[i++]);
-  d.pf_cols = (const int32_t*)(base + offs[i++]);
-  d.pf_map = (const int32_t*)(base + offs[i++]);
+  d.arow_ptr = (const int32_t*)(base + offs[i++]);
+  d.arow = (const int2*)(base + offs[i++]);
+  d.acol_ptr = (const int32_t*)(base + offs[i++]);
+  d.acol = (const int2*)(base + offs[i++]);
+  d.prow_ptr = (const int32_t*)(base + offs[i++]);
+  d.prow = (const int2*)(base + offs[i++]);
   plan->kkt = kkt;
   *out = plan;
   return WMPC_OK;
@@ -322,6 +423,13 @@ int wmpc_osqp_plan_destroy(wmpc_osqp_plan* plan) {
   return WMPC_OK;
 }
 
+int64_t wmpc_osqp_work_floats(const wmpc_osqp_plan* plan, int32_t factor_mode, int64_t ld) {
+  if (!plan || ld < 0 || factor_mode < 0 || factor_mode > 2) return -1;
+  const int64_t nk = plan->kkt->host.n, nl = plan->kkt->host.nnz_l;
+  int64_t cols = factor_mode == 2 ? ld : (factor_mode == 1 ? 1 : 0);
+  return 2 * nk * ld + (nl + nk) * cols;
+}
+
 int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
                          const float* a_values, int64_t pm_ld, float* lx, float* dinv,
                          int64_t batch, int64_t ld, void* stream) {
@@ -331,64 +439,96 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
   WMPC_REQUIRE(pm_ld >= batch || (pm_ld == 0 && batch <= 1),
                "pm_ld must be >= batch, or 0 with batch 1 (one shared matrix, one factor)");
   if (batch == 0) return WMPC_OK;
-  const QdldlTables& t = plan->kkt->t;
+  const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
-  const int64_t grid = (batch + kOsqpBlock - 1) / kOsqpBlock;
+  const size_t lds = factor_lds(P);
+  int rc = osqp_prepare_lds(k_osqp_factor, lds);
+  if (rc) return rc;
+  const int64_t grid = (batch + kWave - 1) / kWave;
   KktSource src{p_values, a_values, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-  hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)grid), dim3(kOsqpBlock),
-                     (size_t)t.max_reach * kOsqpBlock * sizeof(float), (hipStream_t)stream, t, src,
-                     lx, nullptr, dinv, batch, ld);
-  return osqp_check_launch("osqp factor");
+  hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream, P,
+                     src, lx, dinv, batch, ld);
+  return check_launch("osqp factor");
 }
 
 int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         const float* a_values, int64_t pm_ld, const float* q, const float* l,
                         const float* u, const float* x0, const float* z0, const float* y0,
-                        const float* lx, const float* dinv, int64_t f_ld, float* lx_work,
-                        float* dinv_work, float* work, float* x, float* z, float* y, float* prim,
-                        float* dual, float* obj, int32_t max_iter, int64_t batch, int64_t ld,
-                        void* stream) {
+                        const float* lx, const float* dinv, int64_t f_ld, float* work,
+                        int64_t work_len, float* x, float* z, float* y, float* prim, float* dual,
+                        float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && q && l && u && work && x && z && y && prim && dual && obj)), "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && q && l && u && work && x && z && y &&
+                                       prim && dual && obj)),
+               "null argument");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   WMPC_REQUIRE(pm_ld == 0 || pm_ld >= batch, "pm_ld must be 0 (shared) or >= batch");
   WMPC_REQUIRE(max_iter >= 0, "max_iter must be nonnegative");
   if (batch == 0) return WMPC_OK;
-  const QdldlTables& t = plan->kkt->t;
+  WMPC_REQUIRE((lx == nullptr) == (dinv == nullptr), "lx and dinv must be given together");
+  const int32_t mode = lx ? 0 : (pm_ld == 0 ? 1 : 2);
+  WMPC_REQUIRE(work_len >= wmpc_osqp_work_floats(plan, mode, ld),
+               "work is smaller than wmpc_osqp_work_floats(plan, mode, ld)");
+  const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
-  const int64_t grid = (batch + kOsqpBlock - 1) / kOsqpBlock;
-  const size_t lds = (size_t)t.max_reach * kOsqpBlock * sizeof(float);
+  const int64_t nk = P.n, nl = P.nnz_l;
+  const int64_t grid = (batch + kWave - 1) / kWave;
   hipStream_t st = (hipStream_t)stream;
-  if (lx || dinv) {
-    WMPC_REQUIRE(lx && dinv, "lx and dinv must be given together");
+  float* stash = work;
+  float* rbuf = work + nk * ld;
+  float* fwork = rbuf + nk * ld;
+  if (mode == 0) {
     WMPC_REQUIRE(f_ld == 0 || f_ld >= batch, "f_ld must be 0 (shared) or >= batch");
-    if (f_ld == 0)
-      hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kOsqpBlock), 0, st, t,
+    const bool sh = f_ld == 0;
+    const size_t lds = admm_lds(P, sh, false);
+    const int32_t slots = admm_lane_slots(P, sh, false);
+    if (sh) {
+      int rc = osqp_prepare_lds(k_osqp_admm<true, false>, lds);
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kWave), lds, st, P,
                          dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, (float*)lx,
-                         (float*)dinv, (int64_t)0, work, x, z, y, prim, dual, obj, max_iter, batch, ld);
-    else
-      hipLaunchKernelGGL((k_osqp_admm<false, false>), dim3((unsigned)grid), dim3(kOsqpBlock), 0, st,
-                         t, dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, (float*)lx,
-                         (float*)dinv, f_ld, work, x, z, y, prim, dual, obj, max_iter, batch, ld);
-    return osqp_check_launch("osqp admm");
+                         (float*)dinv, (int64_t)0, stash, rbuf, x, z, y, prim, dual, obj, max_iter,
+                         batch, ld, slots);
+    } else {
+      int rc = osqp_prepare_lds(k_osqp_admm<false, false>, lds);
+      if (rc) return rc;
+      hipLaunchKernelGGL((k_osqp_admm<false, false>), dim3((unsigned)grid), dim3(kWave), lds, st,
+                         P, dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, (float*)lx,
+                         (float*)dinv, f_ld, stash, rbuf, x, z, y, prim, dual, obj, max_iter, batch,
+                         ld, slots);
+    }
+    return check_launch("osqp admm");
   }
-  WMPC_REQUIRE(lx_work && dinv_work, "lx_work/dinv_work required when no factor is given");
-  if (pm_ld == 0) {
-    // one shared matrix: factor it once (instance 0, ld 1), then solve the batch
+  if (mode == 1) {
+    // one shared matrix: factor it once (one lane), then solve the batch
+    float* lxw = fwork;
+    float* dinvw = lxw + nl;
     KktSource src{p_values, a_values, 0, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-    hipLaunchKernelGGL(k_osqp_factor, dim3(1), dim3(kOsqpBlock), lds, st, t, src, lx_work,
-                       nullptr, dinv_work, (int64_t)1, (int64_t)1);
-    int rc = osqp_check_launch("osqp shared factor");
+    const size_t flds = factor_lds(P);
+    int rc = osqp_prepare_lds(k_osqp_factor, flds);
     if (rc) return rc;
-    hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kOsqpBlock), 0, st, t,
-                       dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lx_work, dinv_work,
-                       (int64_t)0, work, x, z, y, prim, dual, obj, max_iter, batch, ld);
-    return osqp_check_launch("osqp admm");
+    hipLaunchKernelGGL(k_osqp_factor, dim3(1), dim3(kWave), flds, st, P, src, lxw, dinvw,
+                       (int64_t)1, (int64_t)1);
+    rc = check_launch("osqp shared factor");
+    if (rc) return rc;
+    const size_t lds = admm_lds(P, true, false);
+    rc = osqp_prepare_lds(k_osqp_admm<true, false>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kWave), lds, st, P,
+                       dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, (int64_t)0,
+                       stash, rbuf, x, z, y, prim, dual, obj, max_iter, batch, ld,
+                       admm_lane_slots(P, true, false));
+    return check_launch("osqp admm");
   }
-  hipLaunchKernelGGL((k_osqp_admm<false, true>), dim3((unsigned)grid), dim3(kOsqpBlock), lds, st, t,
-                     dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lx_work, dinv_work, ld,
-                     work, x, z, y, prim, dual, obj, max_iter, batch, ld);
-  return osqp_check_launch("osqp admm");
+  float* lxw = fwork;
+  float* dinvw = lxw + nl * ld;
+  const size_t lds = admm_lds(P, false, true);
+  int rc = osqp_prepare_lds(k_osqp_admm<false, true>, lds);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_osqp_admm<false, true>), dim3((unsigned)grid), dim3(kWave), lds, st, P, dv,
+                     p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, ld, stash, rbuf, x,
+                     z, y, prim, dual, obj, max_iter, batch, ld, admm_lane_slots(P, false, true));
+  return check_launch("osqp admm");
 }
 
 }  // extern "C"

@@ -1,0 +1,19 @@
+// Host-side definition of the opaque QDLDL plan handle shared by the library's
+// translation units.
+#pragma once
+
+#include <vector>
+
+#include "wmpc_program.h"
+#include "wmpc_qdldl_device.h"
+
+struct wmpc_qdldl_plan {
+  wmpc::QdldlProg prog;    // device programs
+  wmpc::HostProgram host;  // host copy of the same programs
+  // copy of the ragged pattern (re-derivation for another LDS budget)
+  std::vector<std::vector<int32_t>> arrays;
+  wmpc_qdldl_pattern pattern{};
+  void* block = nullptr;
+  int device = 0;
+  size_t lds_budget = wmpc::kDefaultLdsBytes;
+};

@@ -2,15 +2,18 @@
 // of include/warpmpc_hip.h.
 //
 // Replaces the Warp kernels of warpmpc/jax_qdldl/warp_backend.py:41-454 and the
-// launch glue of :583-1076.  Launch shape: one lane per instance, 256-lane
-// workgroups, batch-major data, reach slots of the current row in LDS.
+// launch glue of :583-1076.  Launch shape: one 64-lane wavefront per workgroup,
+// one lane per instance, batch-major data, the working values of the current
+// rows in LDS (see wmpc_program.h), each factor/solve one launch.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
 
-#include "wmpc_qdldl_device.h"
+#include "wmpc_plan.h"
+#include "wmpc_stream.h"
+
 
 namespace wmpc {
 
@@ -18,162 +21,172 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
-constexpr int kBlock = 256;
-
 // ---------------------------------------------------------------------------
 // kernels
 
-__global__ __launch_bounds__(kBlock) void k_qdldl_factor(QdldlTables t, DirectSource src,
-                                                         float* __restrict__ lx,
-                                                         float* __restrict__ d,
-                                                         float* __restrict__ dinv, int64_t batch,
-                                                         int64_t ld) {
-  extern __shared__ float ys[];
-  const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (b >= batch) return;
-  factor_lane<DirectSource, false>(t, src, b, b, ld, lx, d, dinv, ys + threadIdx.x, kBlock,
-                                   ArrayRhs{nullptr, 0}, nullptr);
-}
-
-__global__ __launch_bounds__(kBlock) void k_qdldl_factor_solve(
-    QdldlTables t, DirectSource src, ArrayRhs rhs, float* __restrict__ x, float* __restrict__ lx,
-    float* __restrict__ d, float* __restrict__ dinv, float* __restrict__ w, int64_t batch,
-    int64_t ld) {
-  extern __shared__ float ys[];
-  const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (b >= batch) return;
-  factor_lane<DirectSource, true>(t, src, b, b, ld, lx, d, dinv, ys + threadIdx.x, kBlock, rhs, w);
-  backward_cols<false>(t, lx, dinv, ld, b, ld, w);
-  for (int32_t i = 0; i < t.n; ++i) x[(int64_t)i * ld + b] = w[(int64_t)t.pinv[i] * ld + b];
-}
-
-template <bool SHARED>
-__global__ __launch_bounds__(kBlock) void k_qdldl_solve(QdldlTables t,
-                                                        const float* __restrict__ lx,
-                                                        const float* __restrict__ dinv,
-                                                        int64_t f_ld, ArrayRhs rhs,
-                                                        float* __restrict__ x,
-                                                        float* __restrict__ w, int64_t batch,
+__global__ __launch_bounds__(kWave) void k_qdldl_factor(QdldlProg P, DirectSource src,
+                                                        float* __restrict__ lx,
+                                                        float* __restrict__ d,
+                                                        float* __restrict__ dinv, int64_t batch,
                                                         int64_t ld) {
-  const int64_t b = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  extern __shared__ float lds[];
+  const int64_t b = (int64_t)blockIdx.x * kWave + threadIdx.x;
   if (b >= batch) return;
-  forward_rows<SHARED>(t, lx, f_ld, b, ld, rhs, w);
-  backward_cols<SHARED>(t, lx, dinv, f_ld, b, ld, w);
-  for (int32_t i = 0; i < t.n; ++i) x[(int64_t)i * ld + b] = w[(int64_t)t.pinv[i] * ld + b];
+  factor_rows<false>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
+                     lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
+}
+
+__global__ __launch_bounds__(kWave) void k_qdldl_factor_solve(
+    QdldlProg P, DirectSource src, const float* __restrict__ rhs, float* __restrict__ x,
+    float* __restrict__ lx, float* __restrict__ d, float* __restrict__ dinv,
+    float* __restrict__ w, int64_t batch, int64_t ld, int32_t lane_slots) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
+  // lanes past the batch mirror the last instance (identical loads and stores)
+  // so that every lane takes part in the wave-shared staging
+  const int64_t b = b0 < batch ? b0 : batch - 1;
+  const Strided stash{w, ld, b};
+  factor_rows<true>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
+                    lds + lane, ArrayRhs{CStrided{rhs, ld, b}}, stash);
+  const SweepProg S = sweep_prog(P);
+  bwd_cols<false>(S, lx, ld, b, w, ld, b, lds + lane,
+                  reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave), lane,
+                  ArrayOut{Strided{x, ld, b}});
+}
+
+// SHARED: one factor ([nnz_l], [n]) for the whole batch.
+template <bool SHARED>
+__global__ __launch_bounds__(kWave) void k_qdldl_solve(QdldlProg P, const float* __restrict__ lx,
+                                                       const float* __restrict__ dinv,
+                                                       int64_t f_ld, const float* __restrict__ rhs,
+                                                       float* __restrict__ x,
+                                                       float* __restrict__ w, int64_t batch,
+                                                       int64_t ld, int32_t lane_slots) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
+  const int64_t b = b0 < batch ? b0 : batch - 1;
+  const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
+  // right-hand side in first-touch order, staged in the x rows
+#pragma unroll 8
+  for (int32_t i = 0; i < P.n; ++i) x[(int64_t)i * ld + b] = rhs[(int64_t)P.rperm[i] * ld + b];
+  const SweepProg S = sweep_prog(P);
+  char* wave_lds = reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave);
+  fwd_cols<SHARED>(S, lx, dinv, fs, fb, x, w, ld, b, lds + lane, wave_lds, lane);
+  bwd_cols<SHARED>(S, lx, fs, fb, w, ld, b, lds + lane, wave_lds, lane, ArrayOut{Strided{x, ld, b}});
 }
 
 // ---------------------------------------------------------------------------
 // plan construction (host)
 
-static int build_tables(const wmpc_qdldl_pattern* pt, std::vector<int32_t>& blob,
-                        QdldlTables& t, std::vector<size_t>& offs) {
-  const int32_t n = pt->n;
-  WMPC_REQUIRE(n > 0 && pt->nnz_a > 0 && pt->nnz_l >= 0, "pattern sizes must be positive");
-  WMPC_REQUIRE(pt->perm_to_orig && pt->diag_pos && pt->init_ptr && pt->init_rows &&
-                   pt->init_pos && pt->proc_ptr && pt->proc_cols && pt->proc_lidx &&
-                   pt->proc_off && pt->lp && pt->li && pt->p && pt->pinv,
-               "pattern has a null table");
-  WMPC_REQUIRE(pt->lp[0] == 0 && pt->lp[n] == pt->nnz_l, "lp must span [0, nnz_l]");
-  const int32_t n_init = pt->init_ptr[n];
-  const int32_t n_proc = pt->proc_ptr[n];
-  WMPC_REQUIRE(n_proc == pt->nnz_l, "row programs must produce every entry of L once");
-  std::vector<int32_t> diag_src(n), init_slot(n_init), init_src(n_init), upd_ptr(n_proc + 1);
-  std::vector<int32_t> upd_slot;
-  std::vector<int32_t> slot_of(n, -1), stamp(n, -1);
-  int32_t max_reach = 1;
-  for (int32_t k = 0; k < n; ++k) {
-    const int32_t dp = pt->diag_pos[k];
-    WMPC_REQUIRE(dp >= 0 && dp < pt->nnz_a, "diag_pos out of range");
-    diag_src[k] = pt->perm_to_orig[dp];
-    const int32_t r0 = pt->proc_ptr[k], r1 = pt->proc_ptr[k + 1];
-    WMPC_REQUIRE(r0 <= r1, "proc_ptr must be nondecreasing");
-    max_reach = std::max(max_reach, r1 - r0);
-    for (int32_t r = r0; r < r1; ++r) {
-      const int32_t c = pt->proc_cols[r];
-      WMPC_REQUIRE(c >= 0 && c < k, "reach column must precede its row");
-      slot_of[c] = r - r0;
-      stamp[c] = k;
-    }
-    for (int32_t j = pt->init_ptr[k]; j < pt->init_ptr[k + 1]; ++j) {
-      const int32_t row = pt->init_rows[j];
-      WMPC_REQUIRE(row >= 0 && row < k && stamp[row] == k, "init row outside the reach of its row");
-      init_slot[j] = slot_of[row];
-      const int32_t ip = pt->init_pos[j];
-      WMPC_REQUIRE(ip >= 0 && ip < pt->nnz_a, "init_pos out of range");
-      init_src[j] = pt->perm_to_orig[ip];
-    }
-    for (int32_t r = r0; r < r1; ++r) {
-      const int32_t c = pt->proc_cols[r];
-      const int32_t off = pt->proc_off[r];
-      WMPC_REQUIRE(off >= 0 && pt->lp[c] + off <= pt->lp[c + 1], "proc_off out of range");
-      WMPC_REQUIRE(pt->proc_lidx[r] == pt->lp[c] + off, "proc_lidx must be lp[c] + offset");
-      upd_ptr[r] = (int32_t)upd_slot.size();
-      for (int32_t q = 0; q < off; ++q) {
-        const int32_t row = pt->li[pt->lp[c] + q];
-        WMPC_REQUIRE(row >= 0 && row < k && stamp[row] == k && slot_of[row] > r - r0,
-                     "update target must be a later reach slot");
-        upd_slot.push_back(slot_of[row]);
-      }
-    }
-  }
-  upd_ptr[n_proc] = (int32_t)upd_slot.size();
-  for (int32_t i = 0; i < n; ++i) {
-    WMPC_REQUIRE(pt->p[i] >= 0 && pt->p[i] < n && pt->pinv[pt->p[i]] == i, "p/pinv mismatch");
-  }
-  auto append = [&](const int32_t* src, size_t cnt) {
+static int upload_program(wmpc_qdldl_plan* plan) {
+  const HostProgram& h = plan->host;
+  std::vector<int32_t> blob;
+  std::vector<size_t> offs;
+  auto append = [&](const std::vector<int32_t>& v) {
     offs.push_back(blob.size());
-    blob.insert(blob.end(), src, src + cnt);
+    blob.insert(blob.end(), v.begin(), v.end());
+    blob.push_back(0);                      // keep empty arrays addressable
     while (blob.size() % 64) blob.push_back(0);  // 256-byte aligned sub-arrays
   };
-  append(diag_src.data(), n);
-  append(pt->init_ptr, n + 1);
-  append(init_slot.data(), n_init);
-  append(init_src.data(), n_init);
-  append(pt->proc_ptr, n + 1);
-  append(pt->proc_cols, n_proc);
-  append(pt->proc_lidx, n_proc);
-  append(upd_ptr.data(), n_proc + 1);
-  append(upd_slot.data(), upd_slot.size());
-  append(pt->lp, n + 1);
-  append(pt->li, pt->nnz_l);
-  append(pt->p, n);
-  append(pt->pinv, n);
-  t.n = n;
-  t.nnz_a = pt->nnz_a;
-  t.nnz_l = pt->nnz_l;
-  t.max_reach = max_reach;
-  t.n_proc = n_proc;
-  t.n_upd = (int32_t)upd_slot.size();
-  return WMPC_OK;
-}
-
-static void bind_tables(QdldlTables& t, const int32_t* base, const std::vector<size_t>& offs) {
-  size_t i = 0;
-  t.diag_src = base + offs[i++];
-  t.init_ptr = base + offs[i++];
-  t.init_slot = base + offs[i++];
-  t.init_src = base + offs[i++];
-  t.proc_ptr = base + offs[i++];
-  t.proc_col = base + offs[i++];
-  t.proc_lidx = base + offs[i++];
-  t.upd_ptr = base + offs[i++];
-  t.upd_slot = base + offs[i++];
-  t.lp = base + offs[i++];
-  t.li = base + offs[i++];
-  t.p = base + offs[i++];
-  t.pinv = base + offs[i++];
-}
-
-static int check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
+  append(h.row);
+  append(h.init);
+  append(h.proc);
+  append(h.upd);
+  append(h.bcol);
+  append(h.bent);
+  append(h.p);
+  append(h.fcol);
+  append(h.bcol2);
+  append(h.fent);
+  append(h.rperm);
+  void* dev = nullptr;
+  WMPC_HIP_CHECK(hipMalloc(&dev, blob.size() * sizeof(int32_t)));
+  hipError_t e = hipMemcpy(dev, blob.data(), blob.size() * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    set_error(std::string(what) + " launch: " + hipGetErrorString(e));
+    (void)hipFree(dev);
+    set_error(std::string("plan upload: ") + hipGetErrorString(e));
     return WMPC_ERR_HIP;
   }
+  if (plan->block) WMPC_HIP_CHECK(hipFree(plan->block));
+  plan->block = dev;
+  const int32_t* base = static_cast<const int32_t*>(dev);
+  QdldlProg& P = plan->prog;
+  P.n = h.n;
+  P.nnz_a = h.nnz_a;
+  P.nnz_l = h.nnz_l;
+  P.ymax = h.ymax;
+  P.ncol = h.ncol;
+  P.nact = h.nact;
+  P.nbw = h.nbw;
+  size_t i = 0;
+  P.row = reinterpret_cast<const int4*>(base + offs[i++]);
+  P.init = reinterpret_cast<const int2*>(base + offs[i++]);
+  P.proc = reinterpret_cast<const int4*>(base + offs[i++]);
+  P.upd = reinterpret_cast<const int2*>(base + offs[i++]);
+  P.bcol = reinterpret_cast<const int2*>(base + offs[i++]);
+  P.bent = base + offs[i++];
+  P.p = base + offs[i++];
+  P.fcol = reinterpret_cast<const int4*>(base + offs[i++]);
+  P.bcol2 = reinterpret_cast<const int4*>(base + offs[i++]);
+  P.fent = base + offs[i++];
+  P.rperm = base + offs[i++];
   return WMPC_OK;
 }
 
-static size_t lds_bytes(const QdldlTables& t) { return (size_t)t.max_reach * kBlock * sizeof(float); }
+// Derive the programs for the plan's LDS budget: live-L slots get whatever the
+// windows of the fused factor+solve leave.
+static int derive(wmpc_qdldl_plan* plan, bool upload = true) {
+  HostProgram probe;
+  std::string err = build_program(&plan->pattern, 0, probe);
+  WMPC_REQUIRE(err.empty(), err.c_str());
+  const int64_t slots = (int64_t)(plan->lds_budget / (kWave * sizeof(float)));
+  const int64_t fixed = probe.ymax + 2 * (int64_t)probe.ncol + sweep_wave_bytes() / (kWave * 4) + 1;
+  WMPC_REQUIRE(slots >= fixed, "LDS budget too small for the row windows of this pattern");
+  int32_t cap = (int32_t)std::min<int64_t>(slots - fixed, INT32_MAX);
+  err = build_program(&plan->pattern, cap, plan->host);
+  WMPC_REQUIRE(err.empty(), err.c_str());
+  if (!upload) {
+    plan->prog = QdldlProg();
+    plan->prog.n = plan->host.n;
+    plan->prog.ymax = plan->host.ymax;
+    plan->prog.ncol = plan->host.ncol;
+    plan->prog.nact = plan->host.nact;
+    plan->prog.nbw = plan->host.nbw;
+    plan->prog.nnz_l = plan->host.nnz_l;
+    return WMPC_OK;
+  }
+  return upload_program(plan);
+}
+
+// per-lane LDS slots and total bytes of the kernels
+static int32_t lane_slots_factor(const QdldlProg& P, bool fwd) {
+  int32_t s = factor_lds_slots(P, fwd);
+  if (fwd) s = std::max(s, sweep_lane_slots(sweep_prog(P), false));
+  return s;
+}
+static size_t lds_bytes_factor(const QdldlProg& P, bool fwd) {
+  return (size_t)lane_slots_factor(P, fwd) * kWave * sizeof(float) + (fwd ? sweep_wave_bytes() : 0);
+}
+static int32_t lane_slots_solve(const QdldlProg& P, bool shared) {
+  return sweep_lane_slots(sweep_prog(P), shared);
+}
+static size_t lds_bytes_solve(const QdldlProg& P, bool shared) {
+  return (size_t)lane_slots_solve(P, shared) * kWave * sizeof(float) + sweep_wave_bytes();
+}
+
+template <class K>
+static int prepare_lds(K kernel, size_t bytes) {
+  if (bytes > kMaxLdsBytes) {
+    set_error("pattern needs more LDS than a workgroup can have");
+    return WMPC_ERR_UNSUPPORTED;
+  }
+  if (bytes > 64 * 1024)
+    WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return WMPC_OK;
+}
 
 }  // namespace wmpc
 
@@ -182,33 +195,58 @@ using namespace wmpc;
 extern "C" {
 
 const char* wmpc_last_error(void) { return g_last_error.c_str(); }
-const char* wmpc_version(void) { return "warpmpc_hip 0.1.0 gfx950"; }
+const char* wmpc_version(void) { return "warpmpc_hip 0.2.0 gfx950"; }
 
-int wmpc_qdldl_plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out) {
+static int plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out, bool upload) {
   clear_error();
   WMPC_REQUIRE(pattern && out, "null argument");
   *out = nullptr;
-  std::vector<int32_t> blob;
-  std::vector<size_t> offs;
-  QdldlTables t;
-  int rc = build_tables(pattern, blob, t, offs);
-  if (rc != WMPC_OK) return rc;
-  WMPC_REQUIRE(lds_bytes(t) <= 64 * 1024, "reach too wide for the LDS slot buffer");
+  WMPC_REQUIRE(pattern->n > 0 && pattern->nnz_a > 0 && pattern->nnz_l >= 0,
+               "pattern sizes must be positive");
+  const int32_t* ptrs[] = {pattern->perm_to_orig, pattern->diag_pos, pattern->init_ptr,
+                           pattern->init_rows,    pattern->init_pos, pattern->proc_ptr,
+                           pattern->proc_cols,    pattern->proc_lidx, pattern->proc_off,
+                           pattern->lp,           pattern->li,       pattern->p,
+                           pattern->pinv};
+  for (const int32_t* q : ptrs) WMPC_REQUIRE(q, "pattern has a null table");
+  const int64_t n = pattern->n;
+  const int64_t n_init = pattern->init_ptr[n], n_proc = pattern->proc_ptr[n];
+  WMPC_REQUIRE(n_init >= 0 && n_proc >= 0, "init_ptr/proc_ptr must be nonnegative");
+  const int64_t lens[] = {pattern->nnz_a, n, n + 1, n_init, n_init, n + 1, n_proc, n_proc,
+                          n_proc, n + 1, pattern->nnz_l, n, n};
   auto* plan = new wmpc_qdldl_plan();
-  hipError_t e = hipGetDevice(&plan->device);
-  if (e == hipSuccess) e = hipMalloc(&plan->block, blob.size() * sizeof(int32_t));
-  if (e == hipSuccess)
-    e = hipMemcpy(plan->block, blob.data(), blob.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    set_error(std::string("plan upload: ") + hipGetErrorString(e));
+  plan->arrays.resize(13);
+  for (int i = 0; i < 13; ++i) plan->arrays[i].assign(ptrs[i], ptrs[i] + lens[i]);
+  wmpc_qdldl_pattern& pt = plan->pattern;
+  pt.n = pattern->n;
+  pt.nnz_a = pattern->nnz_a;
+  pt.nnz_l = pattern->nnz_l;
+  const int32_t** dst[] = {&pt.perm_to_orig, &pt.diag_pos, &pt.init_ptr, &pt.init_rows,
+                           &pt.init_pos,     &pt.proc_ptr, &pt.proc_cols, &pt.proc_lidx,
+                           &pt.proc_off,     &pt.lp,       &pt.li,        &pt.p,
+                           &pt.pinv};
+  for (int i = 0; i < 13; ++i) *dst[i] = plan->arrays[i].data();
+  int rc = WMPC_OK;
+  if (upload && hipGetDevice(&plan->device) != hipSuccess) {
+    set_error("no HIP device");
+    rc = WMPC_ERR_HIP;
+  }
+  if (rc == WMPC_OK) rc = derive(plan, upload);
+  if (rc != WMPC_OK) {
     if (plan->block) (void)hipFree(plan->block);
     delete plan;
-    return WMPC_ERR_HIP;
+    return rc;
   }
-  plan->t = t;
-  bind_tables(plan->t, static_cast<const int32_t*>(plan->block), offs);
   *out = plan;
   return WMPC_OK;
+}
+
+int wmpc_qdldl_plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out) {
+  return plan_create(pattern, out, true);
+}
+
+int wmpc_qdldl_plan_create_host(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out) {
+  return plan_create(pattern, out, false);
 }
 
 int wmpc_qdldl_plan_destroy(wmpc_qdldl_plan* plan) {
@@ -219,14 +257,58 @@ int wmpc_qdldl_plan_destroy(wmpc_qdldl_plan* plan) {
   return WMPC_OK;
 }
 
+int wmpc_qdldl_plan_set_lds_budget(wmpc_qdldl_plan* plan, int64_t bytes) {
+  clear_error();
+  WMPC_REQUIRE(plan, "null plan");
+  WMPC_REQUIRE(bytes > 0 && bytes <= (int64_t)kMaxLdsBytes, "LDS budget must be in (0, 160 KiB]");
+  const size_t old = plan->lds_budget;
+  plan->lds_budget = (size_t)bytes;
+  int rc = derive(plan, plan->block != nullptr);
+  if (rc != WMPC_OK) plan->lds_budget = old;
+  return rc;
+}
+
 int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a, int32_t* nnz_l,
                          int32_t* max_reach) {
   clear_error();
   WMPC_REQUIRE(plan, "null plan");
-  if (n) *n = plan->t.n;
-  if (nnz_a) *nnz_a = plan->t.nnz_a;
-  if (nnz_l) *nnz_l = plan->t.nnz_l;
-  if (max_reach) *max_reach = plan->t.max_reach;
+  if (n) *n = plan->host.n;
+  if (nnz_a) *nnz_a = plan->host.nnz_a;
+  if (nnz_l) *nnz_l = plan->host.nnz_l;
+  if (max_reach) *max_reach = plan->host.ymax;
+  return WMPC_OK;
+}
+
+int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len) {
+  clear_error();
+  WMPC_REQUIRE(plan && out && len >= 9, "need a plan and 9 output entries");
+  const HostProgram& h = plan->host;
+  out[0] = h.ymax;
+  out[1] = h.ncol;
+  out[2] = h.nact;
+  out[3] = h.nbw;
+  out[4] = h.act_need;
+  out[5] = h.global_rereads;
+  out[6] = (int64_t)lds_bytes_factor(plan->prog, true);
+  out[7] = (int64_t)lds_bytes_solve(plan->prog, false);
+  out[8] = (int64_t)plan->lds_budget;
+  return WMPC_OK;
+}
+
+int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t* out,
+                            int64_t cap, int64_t* len) {
+  clear_error();
+  WMPC_REQUIRE(plan && len, "null argument");
+  const HostProgram& h = plan->host;
+  const std::vector<int32_t>* tabs[] = {&h.row,  &h.init,  &h.proc, &h.upd,   &h.bcol,
+                                        &h.bent, &h.fcol,  &h.bcol2, &h.fent, &h.ftpos};
+  WMPC_REQUIRE(which >= 0 && which < 10, "table index must be in [0, 10)");
+  const std::vector<int32_t>& t = *tabs[which];
+  *len = (int64_t)t.size();
+  if (out) {
+    WMPC_REQUIRE(cap >= (int64_t)t.size(), "output buffer too small");
+    std::copy(t.begin(), t.end(), out);
+  }
   return WMPC_OK;
 }
 
@@ -234,49 +316,69 @@ int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, fl
                           float* dinv, int64_t batch, int64_t ld, void* stream) {
   clear_error();
   WMPC_REQUIRE(plan && (batch == 0 || (a_values && lx && d && dinv)), "null argument");
+  WMPC_REQUIRE(plan->block, "host-only plan: create it with wmpc_qdldl_plan_create");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   if (batch == 0) return WMPC_OK;
-  const int64_t grid = (batch + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_qdldl_factor, dim3((unsigned)grid), dim3(kBlock), lds_bytes(plan->t),
-                     (hipStream_t)stream, plan->t, DirectSource{a_values, ld}, lx, d, dinv, batch,
-                     ld);
+  const size_t lds = lds_bytes_factor(plan->prog, false);
+  int rc = prepare_lds(k_qdldl_factor, lds);
+  if (rc) return rc;
+  const int64_t grid = (batch + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_qdldl_factor, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream,
+                     plan->prog, DirectSource{a_values, ld}, lx, d, dinv, batch, ld);
   return check_launch("qdldl factor");
 }
 
 int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t lx_ld,
                          const float* dinv, int64_t d_ld, const float* rhs, float* x, float* work,
-                         int64_t batch, int64_t ld, void* stream) {
+                         int64_t work_len, int64_t batch, int64_t ld, void* stream) {
   clear_error();
   WMPC_REQUIRE(plan && (batch == 0 || (lx && dinv && rhs && x && work)), "null argument");
+  WMPC_REQUIRE(plan->block, "host-only plan: create it with wmpc_qdldl_plan_create");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   WMPC_REQUIRE((lx_ld == 0 && d_ld == 0) || (lx_ld >= batch && d_ld >= batch),
                "factor strides must both be 0 (shared) or both >= batch");
   WMPC_REQUIRE(lx_ld == d_ld, "lx and dinv must share one instance stride");
   if (batch == 0) return WMPC_OK;
-  const int64_t grid = (batch + kBlock - 1) / kBlock;
-  if (lx_ld == 0)
-    hipLaunchKernelGGL(k_qdldl_solve<true>, dim3((unsigned)grid), dim3(kBlock), 0,
-                       (hipStream_t)stream, plan->t, lx, dinv, (int64_t)0, ArrayRhs{rhs, ld}, x,
-                       work, batch, ld);
-  else
-    hipLaunchKernelGGL(k_qdldl_solve<false>, dim3((unsigned)grid), dim3(kBlock), 0,
-                       (hipStream_t)stream, plan->t, lx, dinv, lx_ld, ArrayRhs{rhs, ld}, x, work,
-                       batch, ld);
+  WMPC_REQUIRE(work_len >= (int64_t)plan->prog.n * ld, "work must hold n * ld floats");
+  const bool shared = lx_ld == 0;
+  const size_t lds = lds_bytes_solve(plan->prog, shared);
+  const int32_t slots = lane_slots_solve(plan->prog, shared);
+  const int64_t grid = (batch + kWave - 1) / kWave;
+  WMPC_REQUIRE(x != rhs, "x must not alias rhs (x stages the right-hand side)");
+  if (shared) {
+    int rc = prepare_lds(k_qdldl_solve<true>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_qdldl_solve<true>, dim3((unsigned)grid), dim3(kWave), lds,
+                       (hipStream_t)stream, plan->prog, lx, dinv, (int64_t)0, rhs, x, work, batch,
+                       ld, slots);
+  } else {
+    int rc = prepare_lds(k_qdldl_solve<false>, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_qdldl_solve<false>, dim3((unsigned)grid), dim3(kWave), lds,
+                       (hipStream_t)stream, plan->prog, lx, dinv, lx_ld, rhs, x, work, batch, ld,
+                       slots);
+  }
   return check_launch("qdldl solve");
 }
 
 int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_values,
                                 const float* rhs, float* x, float* lx, float* d, float* dinv,
-                                float* work, int64_t batch, int64_t ld, void* stream) {
+                                float* work, int64_t work_len, int64_t batch, int64_t ld,
+                                void* stream) {
   clear_error();
-  WMPC_REQUIRE(plan && (batch == 0 || (a_values && rhs && x && lx && d && dinv && work)), "null argument");
+  WMPC_REQUIRE(plan && (batch == 0 || (a_values && rhs && x && lx && d && dinv && work)),
+               "null argument");
+  WMPC_REQUIRE(plan->block, "host-only plan: create it with wmpc_qdldl_plan_create");
   WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
   if (batch == 0) return WMPC_OK;
-  const int64_t grid = (batch + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_qdldl_factor_solve, dim3((unsigned)grid), dim3(kBlock),
-                     lds_bytes(plan->t), (hipStream_t)stream, plan->t,
-                     DirectSource{a_values, ld}, ArrayRhs{rhs, ld}, x, lx, d, dinv, work, batch,
-                     ld);
+  WMPC_REQUIRE(work_len >= (int64_t)plan->prog.n * ld, "work must hold n * ld floats");
+  const size_t lds = lds_bytes_factor(plan->prog, true);
+  int rc = prepare_lds(k_qdldl_factor_solve, lds);
+  if (rc) return rc;
+  const int64_t grid = (batch + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_qdldl_factor_solve, dim3((unsigned)grid), dim3(kWave), lds,
+                     (hipStream_t)stream, plan->prog, DirectSource{a_values, ld}, rhs, x, lx, d,
+                     dinv, work, batch, ld, lane_slots_factor(plan->prog, true));
   return check_launch("qdldl factor+solve");
 }
 

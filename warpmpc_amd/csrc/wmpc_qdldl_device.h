@@ -1,26 +1,54 @@
 // Device-side building blocks of the batched fixed-pattern LDLᵀ (gfx950).
 //
-// One lane owns one instance.  Every per-instance array is batch-major
-// ([entry][ld]), so a wavefront touching entry i of 64 consecutive instances
-// issues one contiguous 256-byte access.  All table reads are wave-uniform and
-// compile to scalar (SMEM) loads.
+// One lane owns one instance; one wavefront (64 lanes) is one workgroup.  Every
+// per-instance array in global memory is batch-major ([entry][ld]), so a
+// wavefront touching entry i of 64 consecutive instances issues one contiguous
+// 256-byte access.  Program tables are wave-uniform (scalar loads).  Working
+// values live in LDS slots laid out [slot][64]: lane t of slot s is
+// lds[s * 64 + t], so one slot access of a wavefront hits 64 distinct banks.
 //
-// The row program follows the up-looking QDLDL factorization of
-// warpmpc/jax_qdldl/core.py:867-933 (== warp_backend.py:104-173); the dense
-// scratch vector y of the reference is replaced by the R <= max_reach slots of
-// the reach of the current row, kept in LDS.
+// The factorization is the up-looking QDLDL recurrence of
+// warpmpc/jax_qdldl/core.py:867-933 (== warp_backend.py:41-173); the solves are
+// x = Pᵀ L⁻ᵀ D⁻¹ L⁻¹ P b (core.py:935-1028) with the forward substitution in
+// row (gather) form and the backward substitution in column (gather) form.
 #pragma once
 
 #include "wmpc_common.h"
 
 namespace wmpc {
 
+constexpr int kWave = 64;
+
+// Device view of one pattern's programs (see wmpc_program.h for their meaning).
+struct QdldlProg {
+  int32_t n = 0, nnz_a = 0, nnz_l = 0;
+  int32_t ymax = 0, ncol = 0, nact = 0, nbw = 0;
+  const int4* row = nullptr;   // [n] {proc_end, init_end, colslot_k, diag_src}
+  const int2* init = nullptr;  // {yslot, src}
+  const int4* proc = nullptr;  // {colslot(c), lidx, AL slot kept or -1, upd_end}
+  const int2* upd = nullptr;   // {yslot, AL slot or -(lidx+1)}
+  const int2* bcol = nullptr;  // [n] {bslot_c or -1, lp[c+1]}
+  const int32_t* bent = nullptr;  // [nnz_l]
+  const int32_t* p = nullptr;     // [n]
+  // column sweeps (wmpc_stream.h)
+  const int4* fcol = nullptr;     // [n]
+  const int4* bcol2 = nullptr;    // [n]
+  const int32_t* fent = nullptr;  // [nnz_l]
+  const int32_t* rperm = nullptr; // [n] original index of the i-th first-touch row
+};
+
+// LDS floats per lane of the factorization (fused forward adds one window).
+__host__ __device__ inline int32_t factor_lds_slots(const QdldlProg& P, bool fwd) {
+  return P.ymax + P.ncol * (fwd ? 2 : 1) + P.nact;
+}
+__host__ __device__ inline int32_t solve_lds_slots(const QdldlProg& P) { return P.ncol + P.nbw; }
+
 // Values straight from a [nnz_a][ld] array (CompiledQDLDL.factor input).
 struct DirectSource {
   const float* __restrict__ a;
   int64_t ld;
   __device__ __forceinline__ float operator()(int32_t slot, int64_t b) const {
-    return a[(int64_t)slot * ld + b];
+    return (a + (int64_t)slot * ld)[(uint32_t)b];
   }
 };
 
@@ -37,94 +65,163 @@ struct KktSource {
   const float* __restrict__ off;
   __device__ __forceinline__ float operator()(int32_t slot, int64_t b) const {
     const int32_t k = kind[slot];
-    const int64_t i = pm_ld ? (int64_t)idx[slot] * pm_ld + b : (int64_t)idx[slot];
-    if (k == 0) return p[i] * scale[slot] + off[slot];
-    if (k == 1) return a[i] * scale[slot];
+    const int64_t row = pm_ld ? (int64_t)idx[slot] * pm_ld : (int64_t)idx[slot];
+    const uint32_t lane = pm_ld ? (uint32_t)b : 0u;
+    if (k == 0) return (p + row)[lane] * scale[slot] + off[slot];
+    if (k == 1) return (a + row)[lane] * scale[slot];
     return off[slot];
   }
 };
 
-// Factor one instance: values of source instance sb, outputs at instance b.
-// y: this lane's LDS slot column (stride ys floats).  d may be null.
-// FUSE_FWD additionally runs the row-oriented forward substitution
-// w[k] = rhs_perm[k] - sum_c L[k,c] w[c] with the freshly computed L[k, :].
-template <class Src, bool FUSE_FWD, class Rhs>
-__device__ __forceinline__ void factor_lane(const QdldlTables& t, const Src& src, int64_t sb,
-                                            int64_t b, int64_t ld, float* __restrict__ lx,
-                                            float* __restrict__ d, float* __restrict__ dinv,
-                                            float* y, int ys, const Rhs& rhs,
-                                            float* __restrict__ w) {
-  for (int32_t k = 0; k < t.n; ++k) {
-    const int32_t r0 = t.proc_ptr[k];
-    const int32_t r1 = t.proc_ptr[k + 1];
-    for (int32_t s = 0; s < r1 - r0; ++s) y[s * ys] = 0.0f;
-    for (int32_t j = t.init_ptr[k]; j < t.init_ptr[k + 1]; ++j)
-      y[t.init_slot[j] * ys] += src(t.init_src[j], sb);
-    float dk = src(t.diag_src[k], sb);
-    float fw = 0.0f;
-    if constexpr (FUSE_FWD) fw = rhs(t.p[k], b);
-    for (int32_t r = r0; r < r1; ++r) {
-      const int32_t c = t.proc_col[r];
-      const int32_t u0 = t.upd_ptr[r];
-      const int32_t u1 = t.upd_ptr[r + 1];
-      const int32_t base = t.lp[c] - u0;
-      const float val = y[(r - r0) * ys];
-      for (int32_t u = u0; u < u1; ++u) {
-        const float l = lx[(int64_t)(base + u) * ld + b];
-        y[t.upd_slot[u] * ys] -= l * val;
-      }
-      const float lval = val * dinv[(int64_t)c * ld + b];
-      lx[(int64_t)t.proc_lidx[r] * ld + b] = lval;
-      dk -= val * lval;
-      if constexpr (FUSE_FWD) fw -= lval * w[(int64_t)c * ld + b];
-    }
-    if (d) d[(int64_t)k * ld + b] = dk;
-    dinv[(int64_t)k * ld + b] = 1.0f / dk;
-    if constexpr (FUSE_FWD) w[(int64_t)k * ld + b] = fw;
-  }
-}
-
-// Forward substitution by rows (gather form): w[k] = rhs_perm[k] - sum L[k,c] w[c].
-// The entries of row k of L are exactly the proc entries of row k.
-// SHARED: one factor for the whole batch, stored as contiguous [nnz_l] / [n].
-template <bool SHARED, class Rhs>
-__device__ __forceinline__ void forward_rows(const QdldlTables& t, const float* __restrict__ lx,
-                                             int64_t f_ld, int64_t b, int64_t ld, const Rhs& rhs,
-                                             float* __restrict__ w) {
-  const int64_t fb = SHARED ? 0 : b;
-  const int64_t fs = SHARED ? 1 : f_ld;
-  for (int32_t k = 0; k < t.n; ++k) {
-    float acc = rhs(t.p[k], b);
-    for (int32_t r = t.proc_ptr[k]; r < t.proc_ptr[k + 1]; ++r) {
-      const float l = lx[(int64_t)t.proc_lidx[r] * fs + fb];
-      acc -= l * w[(int64_t)t.proc_col[r] * ld + b];
-    }
-    w[(int64_t)k * ld + b] = acc;
-  }
-}
-
-// D⁻¹ scaling fused with the backward column sweep (core.py:1003-1028):
-// w[c] = w[c] * dinv[c] - sum_q L[q] w[li[q]], c = n-1 .. 0.
-template <bool SHARED>
-__device__ __forceinline__ void backward_cols(const QdldlTables& t, const float* __restrict__ lx,
-                                              const float* __restrict__ dinv, int64_t f_ld,
-                                              int64_t b, int64_t ld, float* __restrict__ w) {
-  const int64_t fb = SHARED ? 0 : b;
-  const int64_t fs = SHARED ? 1 : f_ld;
-  for (int32_t c = t.n - 1; c >= 0; --c) {
-    float corr = 0.0f;
-    for (int32_t q = t.lp[c]; q < t.lp[c + 1]; ++q)
-      corr += lx[(int64_t)q * fs + fb] * w[(int64_t)t.li[q] * ld + b];
-    w[(int64_t)c * ld + b] = w[(int64_t)c * ld + b] * dinv[(int64_t)c * fs + fb] - corr;
-  }
-}
-
-struct ArrayRhs {
-  const float* __restrict__ rhs;
+// Strided per-instance array: element i of instance b at ptr[i * ld + b]
+// (ld == 1, b == 0: one array shared by the whole batch).  The address is
+// formed as a wave-uniform row base (i * ld, scalar registers) plus the lane's
+// 32-bit offset b, the form the scalar-base global loads take directly.
+struct Strided {
+  float* ptr;
   int64_t ld;
-  __device__ __forceinline__ float operator()(int32_t i, int64_t b) const {
-    return rhs[(int64_t)i * ld + b];
+  int64_t b;
+  __device__ __forceinline__ float& operator()(int64_t i) const {
+    return (ptr + i * ld)[(uint32_t)b];
   }
 };
+struct CStrided {
+  const float* __restrict__ ptr;
+  int64_t ld;
+  int64_t b;
+  __device__ __forceinline__ float operator()(int64_t i) const {
+    return (ptr + i * ld)[(uint32_t)b];
+  }
+};
+
+struct NoRhs {
+  __device__ __forceinline__ float operator()(int32_t, int32_t) const { return 0.f; }
+};
+
+// Up-looking factorization of one instance (values src(slot, sb)).
+// lx / d / dinv: outputs (d may have ptr == nullptr).  lds: this lane's slot
+// column base.  FWD additionally runs the forward substitution
+// w[k] = rhs(k) - sum_c L[k,c] w[c] with the freshly computed row of L and
+// stores w into stash(k).
+template <bool FWD, class Src, class Rhs>
+__device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, int64_t sb,
+                                            const Strided& lx, const Strided& d,
+                                            const Strided& dinv, float* lds, const Rhs& rhs,
+                                            const Strided& stash) {
+  float* Y = lds;
+  float* CW = Y + P.ymax * kWave;
+  float* AL = CW + P.ncol * kWave;
+  float* FW = AL + P.nact * kWave;
+  int32_t r = 0, j = 0, u = 0;
+  for (int32_t k = 0; k < P.n; ++k) {
+    const int4 ri = P.row[k];
+    const int32_t r1 = ri.x;
+    for (int32_t s = 0; s < r1 - r; ++s) Y[s * kWave] = 0.f;
+    for (; j < ri.y; ++j) {
+      const int2 e = P.init[j];
+      Y[e.x * kWave] += src(e.y, sb);
+    }
+    float dk = src(ri.w, sb);
+    float fw = 0.f;
+    if constexpr (FWD) fw = rhs(k, P.p[k]);
+    for (int32_t s = 0; r < r1; ++r, ++s) {
+      const int4 pe = P.proc[r];
+      const float val = Y[s * kWave];
+      // the updates of one reach entry hit distinct later slots: issue their
+      // loads in groups so several LDS / global round trips overlap
+      for (; u + 4 <= pe.w; u += 4) {
+        const int2 e0 = P.upd[u], e1 = P.upd[u + 1], e2 = P.upd[u + 2], e3 = P.upd[u + 3];
+        const float l0 = e0.y >= 0 ? AL[e0.y * kWave] : lx(-(int64_t)e0.y - 1);
+        const float l1 = e1.y >= 0 ? AL[e1.y * kWave] : lx(-(int64_t)e1.y - 1);
+        const float l2 = e2.y >= 0 ? AL[e2.y * kWave] : lx(-(int64_t)e2.y - 1);
+        const float l3 = e3.y >= 0 ? AL[e3.y * kWave] : lx(-(int64_t)e3.y - 1);
+        const float y0 = Y[e0.x * kWave], y1 = Y[e1.x * kWave];
+        const float y2 = Y[e2.x * kWave], y3 = Y[e3.x * kWave];
+        Y[e0.x * kWave] = y0 - l0 * val;
+        Y[e1.x * kWave] = y1 - l1 * val;
+        Y[e2.x * kWave] = y2 - l2 * val;
+        Y[e3.x * kWave] = y3 - l3 * val;
+      }
+      for (; u < pe.w; ++u) {
+        const int2 e = P.upd[u];
+        const float l = e.y >= 0 ? AL[e.y * kWave] : lx(-(int64_t)e.y - 1);
+        Y[e.x * kWave] -= l * val;
+      }
+      const float lval = val * CW[pe.x * kWave];
+      lx(pe.y) = lval;
+      if (pe.z >= 0) AL[pe.z * kWave] = lval;
+      dk -= val * lval;
+      if constexpr (FWD) fw -= lval * FW[pe.x * kWave];
+    }
+    if (d.ptr) d(k) = dk;
+    const float di = 1.0f / dk;
+    dinv(k) = di;
+    if (ri.z >= 0) {
+      CW[ri.z * kWave] = di;
+      if constexpr (FWD) FW[ri.z * kWave] = fw;
+    }
+    // the backward sweep consumes the D⁻¹-scaled forward result
+    if constexpr (FWD) stash(k) = fw * di;
+  }
+}
+
+// Forward substitution by rows: w[k] = rhs(k, p[k]) - sum_c L[k,c] w[c], stored
+// to stash(k); FW is the column window (slots of the factor program).
+// (Reference form; the kernels use the prefetched column sweeps of
+// wmpc_stream.h.)
+template <class LAcc, class Rhs>
+__device__ __forceinline__ void forward_rows(const QdldlProg& P, const LAcc& L, const Rhs& rhs,
+                                             const Strided& stash, float* FW) {
+  int32_t r = 0;
+  for (int32_t k = 0; k < P.n; ++k) {
+    const int4 ri = P.row[k];
+    float acc = rhs(k, P.p[k]);
+    for (; r < ri.x; ++r) {
+      const int4 a = P.proc[r];
+      acc -= L(a.y) * FW[a.x * kWave];
+    }
+    if (ri.z >= 0) FW[ri.z * kWave] = acc;
+    stash(k) = acc;
+  }
+}
+
+// D⁻¹ scaling fused with the backward substitution by columns, c = n-1 .. 0:
+// w[c] = stash(c) * dinv(c) - sum_q L[q] w[li[q]]; out(c, p[c], w[c]) receives
+// the solution entry of original index p[c].
+template <class LAcc, class DAcc, class Out>
+__device__ __forceinline__ void backward_cols(const QdldlProg& P, const LAcc& L, const DAcc& dinv,
+                                              const Strided& stash, float* BW, const Out& out) {
+  for (int32_t c = P.n - 1; c >= 0; --c) {
+    const int2 bc = P.bcol[c];
+    int32_t q = c > 0 ? P.bcol[c - 1].y : 0;
+    // same order as core.py:1003-1028: the correction is summed, then
+    // subtracted from the D⁻¹-scaled forward value
+    float corr = 0.f;
+    for (; q < bc.y; ++q) corr += L(q) * BW[P.bent[q] * kWave];
+    const float acc = stash(c) * dinv(c) - corr;
+    if (bc.x >= 0) BW[bc.x * kWave] = acc;
+    out(c, P.p[c], acc);
+  }
+}
+
+// rhs read from a strided array in original order.
+struct ArrayRhs {
+  CStrided rhs;
+  __device__ __forceinline__ float operator()(int32_t, int32_t o) const { return rhs(o); }
+};
+
+// solution written to a strided array in original order.
+struct ArrayOut {
+  Strided x;
+  __device__ __forceinline__ void operator()(int32_t, int32_t o, float v) const { x(o) = v; }
+};
+
+// NaN-propagating max/min with the semantics of jnp.maximum / jnp.minimum.
+__device__ __forceinline__ float nan_max(float a, float b) {
+  return (a != a) ? a : ((b != b) ? b : fmaxf(a, b));
+}
+__device__ __forceinline__ float nan_min(float a, float b) {
+  return (a != a) ? a : ((b != b) ? b : fminf(a, b));
+}
 
 }  // namespace wmpc

@@ -184,8 +184,7 @@ def compile_osqp(
         prim = th.empty((b,), dtype=th.float32, device=dev_)
         dual = th.empty((b,), dtype=th.float32, device=dev_)
         obj = th.empty((b,), dtype=th.float32, device=dev_)
-        work = th.empty((nk + n, b), dtype=th.float32, device=dev_)
-        lx = dinv = lxw = dinvw = None
+        lx = dinv = None
         f_ld = 0
         if factors is not None:
             lx = as_device_f32(factors[0], "lx")
@@ -194,14 +193,15 @@ def compile_osqp(
             if lx.shape[0] != nnz_l or dinv.shape != (nk, fb) or fb not in (1, b):
                 raise ValueError("factor shapes do not match the plan / batch")
             f_ld = 0 if (fb == 1 and b != 1) else b
+            mode = 0
         else:
-            cols = 1 if shared else b
-            lxw = th.empty((nnz_l, cols), dtype=th.float32, device=dev_)
-            dinvw = th.empty((nk, cols), dtype=th.float32, device=dev_)
+            mode = 1 if shared else 2
+        work_len = int(lib.wmpc_osqp_work_floats(dev.handle, mode, b))
+        work = th.empty((max(work_len, 1),), dtype=th.float32, device=dev_)
         _lib.check(
             lib.wmpc_osqp_solve_f32(
                 dev.handle, ptr(p_sym), ptr(a_sym), pm_ld, ptr(q_s), ptr(l_s), ptr(u_s), ptr(x0), ptr(z0), ptr(y0),
-                ptr(lx), ptr(dinv), f_ld, ptr(lxw), ptr(dinvw), ptr(work), ptr(x), ptr(z), ptr(y), ptr(prim),
+                ptr(lx), ptr(dinv), f_ld, ptr(work), work.numel(), ptr(x), ptr(z), ptr(y), ptr(prim),
                 ptr(dual), ptr(obj), int(settings.max_iter), b, b, stream_ptr()),
             "wmpc_osqp_solve_f32")
         return x.t().contiguous(), z.t().contiguous(), y.t().contiguous(), prim, dual, obj

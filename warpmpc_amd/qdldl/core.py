@@ -145,7 +145,7 @@ def compile_qdldl_variant(
         x = th.empty((n, b), dtype=th.float32, device=r.device)
         work = th.empty((n, b), dtype=th.float32, device=r.device)
         _lib.check(lib.wmpc_qdldl_solve_f32(dev.handle, ptr(lx), f_ld, ptr(dinv), f_ld, ptr(r), ptr(x), ptr(work),
-                                            b, b, stream_ptr()), "wmpc_qdldl_solve_f32")
+                                            work.numel(), b, b, stream_ptr()), "wmpc_qdldl_solve_f32")
         return x
 
     def factor_and_solve(a_values, rhs):
@@ -162,7 +162,8 @@ def compile_qdldl_variant(
         dinv = th.empty((n, b), dtype=th.float32, device=a.device)
         work = th.empty((n, b), dtype=th.float32, device=a.device)
         _lib.check(lib.wmpc_qdldl_factor_solve_f32(dev.handle, ptr(a), ptr(r), ptr(x), ptr(lx), ptr(d), ptr(dinv),
-                                                   ptr(work), b, b, stream_ptr()), "wmpc_qdldl_factor_solve_f32")
+                                                   ptr(work), work.numel(), b, b, stream_ptr()),
+                   "wmpc_qdldl_factor_solve_f32")
         return x, lx, d
 
     return CompiledQDLDL(
