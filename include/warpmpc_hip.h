@@ -82,7 +82,8 @@ int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a
  * fused factor+solve, LDS bytes of the solve, LDS budget.  len >= 9. */
 int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len);
 /* Host copy of one program table (which: 0 row, 1 init, 2 proc, 3 upd, 4 bcol,
- * 5 bent, 6 fcol, 7 bcol2, 8 fent, 9 ftpos); *len receives its length, out
+ * 5 bent, 6 fcol, 7 bcol2, 8 fent, 9 ftpos, 10 frec, 11 brec); *len receives
+ * its length, out
  * (capacity cap) its contents if non-NULL.  For inspection and CPU replay of
  * the programs. */
 int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t* out, int64_t cap,

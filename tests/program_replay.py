@@ -24,6 +24,7 @@ class HostPlan:
     """A host-only ``wmpc_qdldl_plan`` and its program tables."""
 
     def __init__(self, plan, lds_budget: int | None = None):
+        self.handle = None
         lib = _lib.load()
         self.plan = plan
         self.tables = ragged_program(plan)
@@ -48,9 +49,11 @@ class HostPlan:
         self.bcol2 = _lib.plan_program(h, "bcol2").reshape(-1, 4)
         self.fent = _lib.plan_program(h, "fent")
         self.ftpos = _lib.plan_program(h, "ftpos")
+        self.frec = _lib.plan_program(h, "frec").reshape(-1, 64, 2)
+        self.brec = _lib.plan_program(h, "brec").reshape(-1, 64, 2)
 
     def close(self):
-        if self.handle is not None and self.handle.value:
+        if getattr(self, "handle", None) is not None and self.handle.value:
             _lib.load().wmpc_qdldl_plan_destroy(self.handle)
             self.handle = None
 
@@ -126,6 +129,112 @@ def replay_column_sweeps(hp: HostPlan, lx, dinv, rhs_orig):
         if ws >= 0:
             W[ws] = w
         x[pc] = w
+    return x
+
+
+MARK = 1 << 20
+NO_SLOT = 0xFFFF
+V_PER_CHUNK = 16
+L_PER_CHUNK = 64
+
+
+def replay_record_sweeps(hp: HostPlan, lx, dinv, rhs_orig):
+    """``fwd_cols`` + ``bwd_cols`` over the chunked record streams.
+
+    Each chunk's operands are taken from windows [base, base + 16) (values)
+    and [base, base + 64) (L) exactly as the kernel stages them, so a record
+    that consumes a value outside its chunk's staged window fails here.
+    """
+    n, nl = hp.plan.n, hp.plan.nnz_l
+    p = hp.tables["p"]
+    R = np.zeros(n)
+    R[hp.ftpos] = rhs_orig[p]
+
+    def bases(recs, first):
+        nchunks = recs.shape[0]
+        out = [(0, 0, 0), tuple(first)]
+        for t in range(nchunks - 2):
+            h0, h1 = recs[t][0], recs[t][1]
+            out.append((int(h0[0]), int(h0[1]), int(h1[0])))
+        return out
+
+    # first-chunk offsets are not exported: recompute from the record counts
+    def first_offsets(recs, lfn, vfn, dfn):
+        r = recs[0]
+        nrec = int(r[1][1])
+        l = v = d = 0
+        for k in range(2, 2 + nrec):
+            l += lfn(r[k]); v += vfn(r[k]); d += dfn(r[k])
+        return (l, v, d)
+
+    is_mark = lambda r: bool(int(r[0]) & MARK)
+    f_first = first_offsets(hp.frec, lambda r: 0 if is_mark(r) else 1,
+                            lambda r: (int(int(r[0]) & 0xFFFF) == NO_SLOT) if is_mark(r) else (int(r[0]) >> 16) & 1,
+                            lambda r: 1 if is_mark(r) else 0)
+    W = _Region("W", hp.slots["backwin"])
+    stash = np.zeros(n)
+    li = vi = di = col = 0
+    val = 0.0
+    fb = bases(hp.frec, f_first)
+    for t in range(hp.frec.shape[0]):
+        lb, vb, db = fb[t]
+        chunk = hp.frec[t]
+        nrec = int(chunk[1][1])
+        for k in range(2, 2 + nrec):
+            w = int(chunk[k][0])
+            if w & MARK:
+                slot = w & 0xFFFF
+                if slot != NO_SLOT:
+                    val = W[slot]
+                else:
+                    assert 0 <= vi - vb < V_PER_CHUNK
+                    val = R[vi]
+                    vi += 1
+                assert 0 <= di - db < V_PER_CHUNK
+                stash[col] = val * dinv[di]
+                di += 1
+                col += 1
+            else:
+                assert 0 <= li - lb < L_PER_CHUNK
+                slot = w & 0xFFFF
+                if w & FIRST_TOUCH:
+                    assert 0 <= vi - vb < V_PER_CHUNK
+                    W[slot] = R[vi] - lx[li] * val
+                    vi += 1
+                else:
+                    W[slot] = W[slot] - lx[li] * val
+                li += 1
+    assert (li, vi, di, col) == (nl, n, n, n)
+    b_first = first_offsets(hp.brec, lambda r: 0 if is_mark(r) else 1, lambda r: 1 if is_mark(r) else 0,
+                            lambda r: 0)
+    W = _Region("W", hp.slots["backwin"])
+    x = np.zeros(n)
+    li = vi = 0
+    col = n - 1
+    corr = 0.0
+    bb = bases(hp.brec, b_first)
+    for t in range(hp.brec.shape[0]):
+        lb, vb, _ = bb[t]
+        chunk = hp.brec[t]
+        nrec = int(chunk[1][1])
+        for k in range(2, 2 + nrec):
+            w = int(chunk[k][0])
+            if w & MARK:
+                assert 0 <= vi - vb < V_PER_CHUNK
+                wv = stash[n - 1 - vi] - corr
+                vi += 1
+                corr = 0.0
+                slot = w & 0xFFFF
+                if slot != NO_SLOT:
+                    W[slot] = wv
+                assert int(chunk[k][1]) == p[col]
+                x[p[col]] = wv
+                col -= 1
+            else:
+                assert 0 <= li - lb < L_PER_CHUNK
+                corr += lx[nl - 1 - li] * W[w & 0xFFFF]
+                li += 1
+    assert (li, vi, col) == (nl, n, -1)
     return x
 
 
@@ -223,4 +332,5 @@ def replay_solve(hp: HostPlan, lx, dinv, rhs):
     return replay_backward(hp, lx, dinv, stash)
 
 
-__all__ = ["HostPlan", "replay_backward", "replay_column_sweeps", "replay_factor_solve", "replay_solve"]
+__all__ = ["HostPlan", "replay_backward", "replay_column_sweeps", "replay_factor_solve", "replay_record_sweeps",
+           "replay_solve"]

@@ -111,7 +111,8 @@ SIGNATURES: dict[str, tuple] = {
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
 }
 
-PROGRAM_TABLES = ("row", "init", "proc", "upd", "bcol", "bent", "fcol", "bcol2", "fent", "ftpos")
+PROGRAM_TABLES = ("row", "init", "proc", "upd", "bcol", "bent", "fcol", "bcol2", "fent", "ftpos", "frec",
+                  "brec")
 
 
 class HipError(RuntimeError):

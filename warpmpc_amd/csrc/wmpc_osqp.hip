@@ -128,7 +128,6 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
   // so that every lane takes part in the wave-shared staging of the sweeps
   const int64_t b = b0 < batch ? b0 : batch - 1;
   float* mylds = lds + lane;
-  char* wave_lds = reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave);
   const int64_t pb = pm_ld ? b : 0;
   const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
   const int32_t n = dv.n, m = dv.m;
@@ -150,8 +149,8 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
     const SweepProg SP = sweep_prog(P);
     admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
     for (int32_t it = 0; it < max_iter; ++it) {
-      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, mylds, wave_lds, lane);
-      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, mylds, wave_lds, lane, StashOut{w, ld, b});
+      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, mylds, lane);
+      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, mylds, lane, StashOut{w, ld, b});
       if (it + 1 < max_iter)
         admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
       else

@@ -41,6 +41,22 @@ namespace wmpc {
 // flag bit of fent: this entry is the first update of its row
 constexpr int32_t kFirstTouch = 1 << 16;
 
+// Record streams of the chunked sweeps (wmpc_stream.h).  A record is two
+// int32 words {w, a}; a chunk is kRecChunk records: two header records
+// {l, v}, {d, nrec} giving the stream offsets of the chunk two ahead (L
+// values, per-column values, D⁻¹ values) and this chunk's record count, then
+// up to kRecChunk - 2 records:
+//   entry  : w = W slot of li[q] (| kFirstTouch on its first update), a = 0
+//   marker : w = kRecMark | W slot of row c (kRecNoSlot if none), a = p[c]
+// (forward: the marker of column c precedes its entries; backward: entries
+// in descending q, then the marker of their column).
+constexpr int32_t kRecChunk = 64;
+constexpr int32_t kRecMark = 1 << 20;
+constexpr int32_t kRecNoSlot = 0xFFFF;
+constexpr int32_t kRecSlotMask = 0xFFFF;
+constexpr int32_t kRecVMax = 16;  // per-column values (rhs / D⁻¹ / stash) per chunk
+constexpr int32_t kRecL = 64;     // L values per chunk (>= records per chunk)
+
 struct HostProgram {
   int32_t n = 0, nnz_a = 0, nnz_l = 0;
   int32_t ymax = 0, ncol = 0, nact = 0, nbw = 0, act_need = 0;
@@ -62,7 +78,64 @@ struct HostProgram {
   // ftpos[c] = position of row c in first-touch order.
   std::vector<int32_t> fcol, bcol2, fent, ftpos;
   std::vector<int32_t> rperm;  // rperm[ftpos[c]] = p[c]
+  // chunked record streams (forward, backward) and the offsets of chunk 1
+  std::vector<int32_t> frec, brec;
+  int32_t fchunks = 0, bchunks = 0;
+  int32_t f1[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 };
+
+// Packs a record list into chunks.  v_use[i] / d_use[i]: record i consumes a
+// per-column value / a D⁻¹ value; l_use: it consumes an L value.
+inline void pack_records(const std::vector<int32_t>& rec, const std::vector<uint8_t>& l_use,
+                         const std::vector<uint8_t>& v_use, const std::vector<uint8_t>& d_use,
+                         std::vector<int32_t>& out, int32_t& nchunks, int32_t first[3]) {
+  const int32_t per = kRecChunk - 2;
+  std::vector<int32_t> starts;  // record index where each chunk starts
+  std::vector<int32_t> pl, pv, pd;
+  int32_t l = 0, v = 0, d = 0;
+  size_t i = 0;
+  const size_t nr = rec.size() / 2;
+  while (i < nr) {
+    starts.push_back((int32_t)i);
+    pl.push_back(l);
+    pv.push_back(v);
+    pd.push_back(d);
+    int32_t cnt = 0, cv = 0, cd = 0;
+    while (i < nr && cnt < per && cv + v_use[i] <= kRecVMax && cd + d_use[i] <= kRecVMax) {
+      l += l_use[i];
+      v += v_use[i];
+      d += d_use[i];
+      cv += v_use[i];
+      cd += d_use[i];
+      ++cnt;
+      ++i;
+    }
+  }
+  const int32_t real = (int32_t)starts.size();
+  int32_t total = real + (real & 1) + 2;  // even, plus two empty chunks to prefetch
+  starts.resize(total, (int32_t)nr);
+  pl.resize(total, l);
+  pv.resize(total, v);
+  pd.resize(total, d);
+  out.assign((size_t)total * kRecChunk * 2, 0);
+  for (int32_t t = 0; t < total; ++t) {
+    int32_t* c = &out[(size_t)t * kRecChunk * 2];
+    const int32_t t2 = std::min(t + 2, total - 1);
+    const int32_t b = starts[t], e = t + 1 < total ? starts[t + 1] : (int32_t)nr;
+    c[0] = pl[t2];
+    c[1] = pv[t2];
+    c[2] = pd[t2];
+    c[3] = e - b;
+    for (int32_t k = b; k < e; ++k) {
+      c[2 * (2 + k - b)] = rec[2 * (size_t)k];
+      c[2 * (2 + k - b) + 1] = rec[2 * (size_t)k + 1];
+    }
+  }
+  nchunks = total;
+  first[0] = pl[1];
+  first[1] = pv[1];
+  first[2] = pd[1];
+}
 
 // Free-list slot allocator (lowest free slot first).
 class SlotPool {
@@ -260,6 +333,46 @@ inline std::string build_program(const wmpc_qdldl_pattern* pt, int32_t act_cap, 
   if (pos != n) return "first-touch order does not cover every row";
   hp.rperm.assign(n, 0);
   for (int32_t c = 0; c < n; ++c) hp.rperm[hp.ftpos[c]] = pt->p[c];
+
+  // record streams of the chunked sweeps
+  {
+    std::vector<int32_t> rec;
+    std::vector<uint8_t> lu, vu, du;
+    for (int32_t c = 0; c < n; ++c) {
+      rec.push_back(kRecMark | (bslot[c] >= 0 ? bslot[c] : kRecNoSlot));
+      rec.push_back(pt->p[c]);
+      lu.push_back(0);
+      vu.push_back(bslot[c] >= 0 ? 0 : 1);  // an unopened row takes its rhs here
+      du.push_back(1);
+      for (int32_t q = pt->lp[c]; q < pt->lp[c + 1]; ++q) {
+        rec.push_back(hp.fent[q]);
+        rec.push_back(0);
+        lu.push_back(1);
+        vu.push_back((hp.fent[q] & kFirstTouch) ? 1 : 0);
+        du.push_back(0);
+      }
+    }
+    pack_records(rec, lu, vu, du, hp.frec, hp.fchunks, hp.f1);
+    rec.clear();
+    lu.clear();
+    vu.clear();
+    du.clear();
+    for (int32_t c = n - 1; c >= 0; --c) {
+      for (int32_t q = pt->lp[c + 1] - 1; q >= pt->lp[c]; --q) {
+        rec.push_back(hp.fent[q] & (kFirstTouch - 1));
+        rec.push_back(0);
+        lu.push_back(1);
+        vu.push_back(0);
+        du.push_back(0);
+      }
+      rec.push_back(kRecMark | (bslot[c] >= 0 ? bslot[c] : kRecNoSlot));
+      rec.push_back(pt->p[c]);
+      lu.push_back(0);
+      vu.push_back(1);  // the D⁻¹-scaled forward value of column c
+      du.push_back(0);
+    }
+    pack_records(rec, lu, vu, du, hp.brec, hp.bchunks, hp.b1);
+  }
   return std::string();
 }
 

@@ -17,6 +17,11 @@
 
 namespace wmpc {
 
+static_assert(kRecChunk == kChunkRecs && kRecMark == kMarkBit && kRecNoSlot == kNoSlot &&
+                  kRecSlotMask == kSlotMask && kRecVMax == kVPer && kRecL == kLPer &&
+                  kFirstTouch == kFirstTouchBit,
+              "host record layout must match the device sweeps");
+
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
@@ -50,9 +55,7 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor_solve(
   factor_rows<true>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
                     lds + lane, ArrayRhs{CStrided{rhs, ld, b}}, stash);
   const SweepProg S = sweep_prog(P);
-  bwd_cols<false>(S, lx, ld, b, w, ld, b, lds + lane,
-                  reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave), lane,
-                  ArrayOut{Strided{x, ld, b}});
+  bwd_cols<false>(S, lx, ld, b, w, ld, b, lds + lane, lane, ArrayOut{Strided{x, ld, b}});
 }
 
 // SHARED: one factor ([nnz_l], [n]) for the whole batch.
@@ -72,9 +75,8 @@ __global__ __launch_bounds__(kWave) void k_qdldl_solve(QdldlProg P, const float*
 #pragma unroll 8
   for (int32_t i = 0; i < P.n; ++i) x[(int64_t)i * ld + b] = rhs[(int64_t)P.rperm[i] * ld + b];
   const SweepProg S = sweep_prog(P);
-  char* wave_lds = reinterpret_cast<char*>(lds + (size_t)lane_slots * kWave);
-  fwd_cols<SHARED>(S, lx, dinv, fs, fb, x, w, ld, b, lds + lane, wave_lds, lane);
-  bwd_cols<SHARED>(S, lx, fs, fb, w, ld, b, lds + lane, wave_lds, lane, ArrayOut{Strided{x, ld, b}});
+  fwd_cols<SHARED>(S, lx, dinv, fs, fb, x, w, ld, b, lds + lane, lane);
+  bwd_cols<SHARED>(S, lx, fs, fb, w, ld, b, lds + lane, lane, ArrayOut{Strided{x, ld, b}});
 }
 
 // ---------------------------------------------------------------------------
@@ -101,6 +103,8 @@ static int upload_program(wmpc_qdldl_plan* plan) {
   append(h.bcol2);
   append(h.fent);
   append(h.rperm);
+  append(h.frec);
+  append(h.brec);
   void* dev = nullptr;
   WMPC_HIP_CHECK(hipMalloc(&dev, blob.size() * sizeof(int32_t)));
   hipError_t e = hipMemcpy(dev, blob.data(), blob.size() * sizeof(int32_t), hipMemcpyHostToDevice);
@@ -132,17 +136,31 @@ static int upload_program(wmpc_qdldl_plan* plan) {
   P.bcol2 = reinterpret_cast<const int4*>(base + offs[i++]);
   P.fent = base + offs[i++];
   P.rperm = base + offs[i++];
+  P.frec = base + offs[i++];
+  P.brec = base + offs[i++];
+  P.fchunks = h.fchunks;
+  P.bchunks = h.bchunks;
+  for (int j = 0; j < 3; ++j) {
+    P.f1[j] = h.f1[j];
+    P.b1[j] = h.b1[j];
+  }
   return WMPC_OK;
 }
 
 // Derive the programs for the plan's LDS budget: live-L slots get whatever the
 // windows of the fused factor+solve leave.
-static int derive(wmpc_qdldl_plan* plan, bool upload = true) {
+// A default-budget plan whose windows alone exceed the budget gets the smallest
+// budget (in 4 KiB steps) that holds them; an explicit budget must fit.
+static int derive(wmpc_qdldl_plan* plan, bool upload = true, bool explicit_budget = false) {
   HostProgram probe;
   std::string err = build_program(&plan->pattern, 0, probe);
   WMPC_REQUIRE(err.empty(), err.c_str());
-  const int64_t slots = (int64_t)(plan->lds_budget / (kWave * sizeof(float)));
   const int64_t fixed = probe.ymax + 2 * (int64_t)probe.ncol + sweep_wave_bytes() / (kWave * 4) + 1;
+  if (!explicit_budget) {
+    const size_t need = ((size_t)fixed * kWave * sizeof(float) + 4095) / 4096 * 4096;
+    if (need > plan->lds_budget) plan->lds_budget = std::min(need, kMaxLdsBytes);
+  }
+  const int64_t slots = (int64_t)(plan->lds_budget / (kWave * sizeof(float)));
   WMPC_REQUIRE(slots >= fixed, "LDS budget too small for the row windows of this pattern");
   int32_t cap = (int32_t)std::min<int64_t>(slots - fixed, INT32_MAX);
   err = build_program(&plan->pattern, cap, plan->host);
@@ -208,15 +226,20 @@ static int plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out,
                            pattern->proc_cols,    pattern->proc_lidx, pattern->proc_off,
                            pattern->lp,           pattern->li,       pattern->p,
                            pattern->pinv};
-  for (const int32_t* q : ptrs) WMPC_REQUIRE(q, "pattern has a null table");
   const int64_t n = pattern->n;
+  WMPC_REQUIRE(pattern->init_ptr && pattern->proc_ptr, "pattern has a null table");
   const int64_t n_init = pattern->init_ptr[n], n_proc = pattern->proc_ptr[n];
   WMPC_REQUIRE(n_init >= 0 && n_proc >= 0, "init_ptr/proc_ptr must be nonnegative");
   const int64_t lens[] = {pattern->nnz_a, n, n + 1, n_init, n_init, n + 1, n_proc, n_proc,
                           n_proc, n + 1, pattern->nnz_l, n, n};
+  // an empty table may be passed as NULL
+  for (int i = 0; i < 13; ++i) WMPC_REQUIRE(ptrs[i] || lens[i] == 0, "pattern has a null table");
   auto* plan = new wmpc_qdldl_plan();
   plan->arrays.resize(13);
-  for (int i = 0; i < 13; ++i) plan->arrays[i].assign(ptrs[i], ptrs[i] + lens[i]);
+  for (int i = 0; i < 13; ++i) {
+    if (lens[i] > 0) plan->arrays[i].assign(ptrs[i], ptrs[i] + lens[i]);
+    plan->arrays[i].push_back(0);  // keeps the copy's data pointer valid when empty
+  }
   wmpc_qdldl_pattern& pt = plan->pattern;
   pt.n = pattern->n;
   pt.nnz_a = pattern->nnz_a;
@@ -263,7 +286,7 @@ int wmpc_qdldl_plan_set_lds_budget(wmpc_qdldl_plan* plan, int64_t bytes) {
   WMPC_REQUIRE(bytes > 0 && bytes <= (int64_t)kMaxLdsBytes, "LDS budget must be in (0, 160 KiB]");
   const size_t old = plan->lds_budget;
   plan->lds_budget = (size_t)bytes;
-  int rc = derive(plan, plan->block != nullptr);
+  int rc = derive(plan, plan->block != nullptr, true);
   if (rc != WMPC_OK) plan->lds_budget = old;
   return rc;
 }
@@ -300,9 +323,9 @@ int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t*
   clear_error();
   WMPC_REQUIRE(plan && len, "null argument");
   const HostProgram& h = plan->host;
-  const std::vector<int32_t>* tabs[] = {&h.row,  &h.init,  &h.proc, &h.upd,   &h.bcol,
-                                        &h.bent, &h.fcol,  &h.bcol2, &h.fent, &h.ftpos};
-  WMPC_REQUIRE(which >= 0 && which < 10, "table index must be in [0, 10)");
+  const std::vector<int32_t>* tabs[] = {&h.row,  &h.init, &h.proc,  &h.upd,  &h.bcol,  &h.bent,
+                                        &h.fcol, &h.bcol2, &h.fent, &h.ftpos, &h.frec, &h.brec};
+  WMPC_REQUIRE(which >= 0 && which < 12, "table index must be in [0, 12)");
   const std::vector<int32_t>& t = *tabs[which];
   *len = (int64_t)t.size();
   if (out) {

@@ -35,6 +35,11 @@ struct QdldlProg {
   const int4* bcol2 = nullptr;    // [n]
   const int32_t* fent = nullptr;  // [nnz_l]
   const int32_t* rperm = nullptr; // [n] original index of the i-th first-touch row
+  // chunked record streams of the sweeps and the stream offsets of chunk 1
+  const int32_t* frec = nullptr;
+  const int32_t* brec = nullptr;
+  int32_t fchunks = 0, bchunks = 0;
+  int32_t f1[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
 };
 
 // LDS floats per lane of the factorization (fused forward adds one window).

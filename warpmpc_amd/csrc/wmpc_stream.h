@@ -1,21 +1,16 @@
-// Prefetched streams and the column-oriented triangular sweeps (gfx950).
+// Chunked, prefetched column-oriented triangular sweeps (gfx950).
 //
 // With one lane per instance a batch of 65,536 fills the chip with exactly one
 // wavefront per SIMD, so latency cannot be hidden by other wavefronts: each
-// wavefront must keep many loads in flight itself.  Every operand the sweeps
-// read is therefore consumed as a sequential stream and prefetched:
-//
-//   * LaneStream — a per-lane float stream (L values of one instance, the
-//     right-hand side, the forward result).  Two chunks of CH values are in
-//     flight in VGPRs while two more chunks sit in an LDS ring of the lane's
-//     own column ([slot][64] layout, conflict-free);
-//   * WaveStream — a wave-uniform stream (program tables, a shared factor).
-//     Lane j loads element j of a chunk; the chunk is staged in a wave-shared
-//     LDS ring and read back with a uniform address (an LDS broadcast).
-//
-// A stream element i is readable once the consumer has advanced to it; the
-// consumer reads indices in increasing order, and crossing into a new chunk
-// commits the next staged chunk and issues the loads of the chunk after it.
+// wavefront must keep many loads in flight itself.  A sweep therefore walks a
+// precomputed record stream (column markers and L entries, in chunks of 64
+// records) and every operand it reads is a sequential stream consumed chunk by
+// chunk: the L values of the instance (or of the shared factor), the staged
+// right-hand side / stash values and D⁻¹.  Each chunk's operands are loaded
+// into registers two chunks before use, then committed to LDS (per-lane
+// columns in [slot][64] layout, conflict-free; wave-uniform data in a shared
+// area read by broadcast).  Every chunk issues the same loads unconditionally,
+// so the compiler's memory wait counts stay exact and the prefetch is real.
 #pragma once
 
 #include "wmpc_qdldl_device.h"
@@ -31,355 +26,389 @@ struct I2 {
   int32_t x, y;
 };
 
-template <int CH>
-struct LaneStream {
-  static_assert((CH & (CH - 1)) == 0, "chunk must be a power of two");
-  static constexpr int kSlots = 2 * CH;  // LDS slots of the ring
-  float r0[CH], r1[CH];
-  float* ring;
-  int64_t cur;
-  int64_t count;
-
-  template <class Src>
-  __device__ __forceinline__ void fill(float (&r)[CH], int64_t chunk, const Src& src) {
-    const int64_t base = chunk * CH;
-#pragma unroll
-    for (int j = 0; j < CH; ++j) {
-      const int64_t i = base + j;
-      r[j] = src(i < count ? i : count - 1);
-    }
-  }
-  __device__ __forceinline__ void commit(const float (&r)[CH], int half) {
-#pragma unroll
-    for (int j = 0; j < CH; ++j) ring[(half * CH + j) * kWave] = r[j];
-  }
-  template <class Src>
-  __device__ __forceinline__ void init(float* ring_, int64_t count_, const Src& src) {
-    ring = ring_;
-    count = count_ > 0 ? count_ : 1;
-    cur = 0;
-    fill(r0, 0, src);
-    fill(r1, 1, src);
-    commit(r0, 0);
-    fill(r0, 2, src);
-    commit(r1, 1);
-    fill(r1, 3, src);
-  }
-  // Element i readable with at(i) for every i in [current, i]; crosses at most
-  // one chunk boundary, so a group read spans fewer than CH elements.
-  template <class Src>
-  __device__ __forceinline__ void ensure(int64_t i, const Src& src) {
-    (void)get(i, src);
-  }
-  __device__ __forceinline__ float at(int64_t i) const {
-    return ring[(((i / CH) & 1) * CH + (i & (CH - 1))) * kWave];
-  }
-  template <class Src>
-  __device__ __forceinline__ float get(int64_t i, const Src& src) {
-    if (i >= (cur + 2) * CH) {
-      // chunk cur is consumed: stage chunk cur+2 in its half, load chunk cur+4.
-      // The two arms differ only in which register set they touch; the
-      // markers at both ends keep the compiler from hoisting or sinking their
-      // common code into one path that selects the set through a pointer
-      // (which would demote both sets to scratch memory).
-      if (cur & 1) {
-        asm volatile("; stream set 1 begin");
-        commit(r1, 1);
-        fill(r1, cur + 4, src);
-        asm volatile("; stream set 1 end");
-      } else {
-        asm volatile("; stream set 0 begin");
-        commit(r0, 0);
-        fill(r0, cur + 4, src);
-        asm volatile("; stream set 0 end");
-      }
-      ++cur;
-    }
-    return ring[(((i / CH) & 1) * CH + (i & (CH - 1))) * kWave];
-  }
-};
-
-// Wave-uniform stream of T (4 or 8 or 16 bytes) with 64-element chunks.
+// Global-memory pointers.  Sweep operands are addressed through these so the
+// loads are global_load_* (vector memory only): a generic pointer would give
+// flat loads, which also count against the LDS wait counter and force full
+// waits before every LDS access.
+#define WMPC_GLOBAL __attribute__((address_space(1)))
 template <class T>
-struct WaveStream {
-  static constexpr int CH = kWave;
-  static constexpr int kElems = 2 * CH;  // ring elements (wave-shared)
-  T r0, r1;
-  T* ring;
-  int64_t cur;
-  int64_t count;
-  int lane;
+using gptr = WMPC_GLOBAL T*;
 
-  template <class Src>
-  __device__ __forceinline__ T load(int64_t chunk, const Src& src) const {
-    const int64_t i = chunk * CH + lane;
-    return src(i < count ? i : count - 1);
-  }
-  __device__ __forceinline__ void commit(const T& r, int half) {
-    ring[half * CH + lane] = r;
-    __builtin_amdgcn_wave_barrier();
-  }
-  template <class Src>
-  __device__ __forceinline__ void init(T* ring_, int64_t count_, int lane_, const Src& src) {
-    ring = ring_;
-    count = count_ > 0 ? count_ : 1;
-    lane = lane_;
-    cur = 0;
-    r0 = load(0, src);
-    r1 = load(1, src);
-    commit(r0, 0);
-    r0 = load(2, src);
-    commit(r1, 1);
-    r1 = load(3, src);
-  }
-  template <class Src>
-  __device__ __forceinline__ void ensure(int64_t i, const Src& src) {
-    (void)get(i, src);
-  }
-  __device__ __forceinline__ T at(int64_t i) const {
-    return ring[((i / CH) & 1) * CH + (i & (CH - 1))];
-  }
-  template <class Src>
-  __device__ __forceinline__ T get(int64_t i, const Src& src) {
-    if (i >= (cur + 2) * CH) {
-      if (cur & 1) {
-        asm volatile("; wave stream set 1 begin");
-        commit(r1, 1);
-        r1 = load(cur + 4, src);
-        asm volatile("; wave stream set 1 end");
-      } else {
-        asm volatile("; wave stream set 0 begin");
-        commit(r0, 0);
-        r0 = load(cur + 4, src);
-        asm volatile("; wave stream set 0 end");
-      }
-      ++cur;
-    }
-    return ring[((i / CH) & 1) * CH + (i & (CH - 1))];
-  }
-};
-
-// A wave-uniform pointer the compiler must treat as unknown at each call:
-// keeps the address arithmetic of a sweep's stream set-up inside the sweep
+// A wave-uniform global pointer the compiler must treat as unknown at each
+// call: keeps the address arithmetic of a sweep's set-up inside the sweep
 // instead of being hoisted out of the caller's iteration loop (where the
 // hoisted addresses would stay live across every iteration and spill).
 template <class T>
-__device__ __forceinline__ T* opaque(T* p) {
-  asm volatile("" : "+s"(p));
-  return p;
+__device__ __forceinline__ gptr<T> opaque(const T* p) {
+  gptr<T> g = (gptr<T>)(p);
+  asm volatile("" : "+s"(g));
+  return g;
 }
 
-// Sources of the streams.
-struct LaneArray {  // ptr[i * ld + b]: uniform row base + 32-bit lane offset
-  const float* __restrict__ ptr;
-  int64_t ld, b;
-  __device__ __forceinline__ float operator()(int64_t i) const {
-    return (ptr + i * ld)[(uint32_t)b];
-  }
+// Device view of the chunked record streams (built in wmpc_program.h).
+struct RecStream {
+  const I2* rec = nullptr;  // [nchunks * kChunkRecs]
+  int32_t nchunks = 0;
+  int32_t l1 = 0, v1 = 0, d1 = 0;  // stream offsets of chunk 1
 };
-struct LaneArrayRev {  // ptr[(n - 1 - i) * ld + b]
-  const float* __restrict__ ptr;
-  int64_t ld, b, n;
-  __device__ __forceinline__ float operator()(int64_t i) const {
-    return (ptr + (n - 1 - i) * ld)[(uint32_t)b];
-  }
-};
-
-// Device tables of the column sweeps (see wmpc_program.h).
 struct SweepProg {
   int32_t n = 0, nnz_l = 0, nw = 0;
-  const I4* fcol = nullptr;      // [n]   {W slot of row c or -1, lp[c+1], 0, 0}
-  const I4* bcol = nullptr;      // [n]   reverse: {W slot of row c or -1, lp[c], p[c], 0}
-  const int32_t* fent = nullptr; // [nnz_l] W slot of li[q] | kFirstTouch
+  RecStream fwd, bwd;
 };
 
+constexpr int kChunkRecs = 64;       // records per chunk (2 header + 62)
+constexpr int32_t kMarkBit = 1 << 20;
+constexpr int32_t kSlotMask = 0xFFFF;
+constexpr int32_t kNoSlot = 0xFFFF;
 constexpr int32_t kFirstTouchBit = 1 << 16;
-constexpr int kLChunk = 32;   // L values per lane chunk
-constexpr int kVChunk = 8;    // per-column values per lane chunk
+constexpr int kVPer = 16;            // per-column values per chunk
+constexpr int kLPer = 64;            // L values per chunk
 
 __host__ __device__ inline SweepProg sweep_prog(const QdldlProg& P) {
   SweepProg S;
   S.n = P.n;
   S.nnz_l = P.nnz_l;
   S.nw = P.nbw;
-  S.fcol = reinterpret_cast<const I4*>(P.fcol);
-  S.bcol = reinterpret_cast<const I4*>(P.bcol2);
-  S.fent = P.fent;
+  S.fwd = RecStream{reinterpret_cast<const I2*>(P.frec), P.fchunks, P.f1[0], P.f1[1], P.f1[2]};
+  S.bwd = RecStream{reinterpret_cast<const I2*>(P.brec), P.bchunks, P.b1[0], P.b1[1], P.b1[2]};
   return S;
 }
 
-// LDS floats per lane of the sweeps (W window, L ring unless the factor is
-// shared, right-hand-side / D⁻¹ / stash rings), and the bytes of the
-// wave-shared rings.
+// LDS of the sweeps: per-lane floats (W window, then the current chunk's L
+// values of a per-instance factor and its per-column values); wave-uniform
+// chunk data stays in registers.
 __host__ __device__ inline int32_t sweep_lane_slots(const SweepProg& S, bool shared_l) {
-  return S.nw + (shared_l ? 2 * kVChunk : 2 * kLChunk + 4 * kVChunk);
+  return S.nw + (shared_l ? kVPer : kLPer + 2 * kVPer);
 }
-__host__ __device__ inline int32_t sweep_wave_bytes() {
-  return 2 * kWave * (int32_t)(sizeof(I4) + sizeof(I2));
+__host__ __device__ inline int32_t sweep_wave_bytes() { return 0; }
+
+// One chunk of a sweep in flight: loaded into registers two chunks ahead of
+// use.  Every chunk issues the same loads, so the compiler's wait counts
+// before the chunk is taken over stay exact.
+//   rec, lu, du: wave-uniform data, lane j holding element j (record j, the
+//                shared L value and D⁻¹ value j of the chunk); read back with
+//                readlane into scalar registers;
+//   l, v       : per-lane L values (per-instance factor) and per-column value
+//                streams, committed to the lane's LDS column.
+template <bool SHL, int NV>
+struct Chunk {
+  I2 rec;
+  float lu, du;
+  float l[SHL ? 1 : kLPer];
+  float v[NV][kVPer];
+  int32_t lbase, vbase, dbase;
+};
+
+__device__ __forceinline__ int32_t rl(int32_t v, int32_t k) {
+  return __builtin_amdgcn_readlane(v, k);
+}
+__device__ __forceinline__ float rlf(float v, int32_t k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Forward substitution, column (scatter) form, c = 0 .. n-1:
-//   w[c] is final at column c; stash(c) = w[c] * dinv(c);
-//   w[li[q]] -= L[q] w[c] for the entries of column c, the first update of a
-//   row starting from its right-hand side.
-// rhs: right-hand side in first-touch order, R(i) at rhs[i * ld + b].
-// SHL: one factor shared by the batch (L, dinv wave-uniform).
+template <bool SHL, int NV>
+struct Sweep {
+  float* W;      // lane column: row window
+  float* lring;  // lane column: L values of the current chunk (per-instance factor)
+  float* vring;  // lane column: NV value streams of the current chunk
+  int lane;
+  // the current chunk's wave-uniform data
+  int32_t crx, cry;
+  float clu, cdu;
+
+  __device__ __forceinline__ void setup(float* lane_lds, int32_t nw, int lane_) {
+    lane = lane_;
+    W = lane_lds;
+    lring = W + nw * kWave;
+    vring = lring + (SHL ? 0 : kLPer * kWave);
+  }
+
+  template <class LSrc, class V0Src, class V1Src, class DSrc>
+  __device__ __forceinline__ void issue(Chunk<SHL, NV>& c, gptr<const I2> rec, int32_t t, int32_t lb,
+                                        int32_t vb, int32_t db, const LSrc& ls, const V0Src& v0,
+                                        const V1Src& v1, const DSrc& ds) {
+    const int64_t ri = (int64_t)t * kChunkRecs + lane;
+    c.rec.x = rec[ri].x;  // member-wise: the record lives in global memory
+    c.rec.y = rec[ri].y;
+    c.lbase = lb;
+    c.vbase = vb;
+    c.dbase = db;
+    if constexpr (SHL) {
+      c.lu = ls(lb + lane);
+      c.du = ds(db + lane);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kLPer; ++j) c.l[j] = ls(lb + j);
+    }
+#pragma unroll
+    for (int j = 0; j < kVPer; ++j) c.v[0][j] = v0(vb + j);
+    if constexpr (NV > 1) {
+#pragma unroll
+      for (int j = 0; j < kVPer; ++j) c.v[1][j] = v1(db + j);
+    }
+  }
+  __device__ __forceinline__ void take(const Chunk<SHL, NV>& c) {
+    crx = c.rec.x;
+    cry = c.rec.y;
+    if constexpr (SHL) {
+      clu = c.lu;
+      cdu = c.du;
+    } else {
+#pragma unroll
+      for (int j = 0; j < kLPer; ++j) lring[j * kWave] = c.l[j];
+    }
+#pragma unroll
+    for (int j = 0; j < kVPer; ++j) vring[j * kWave] = c.v[0][j];
+    if constexpr (NV > 1) {
+#pragma unroll
+      for (int j = 0; j < kVPer; ++j) vring[(kVPer + j) * kWave] = c.v[1][j];
+    }
+  }
+  __device__ __forceinline__ int32_t rec_w(int32_t k) const { return rl(crx, k); }
+  __device__ __forceinline__ int32_t rec_a(int32_t k) const { return rl(cry, k); }
+  __device__ __forceinline__ float L(int32_t j) const {
+    if constexpr (SHL) return rlf(clu, j);
+    return lring[j * kWave];
+  }
+  __device__ __forceinline__ float V(int k, int32_t j) const { return vring[(k * kVPer + j) * kWave]; }
+  __device__ __forceinline__ float D(int32_t j) const {
+    if constexpr (SHL) return rlf(cdu, j);
+    return vring[(kVPer + j) * kWave];
+  }
+  // number of consecutive entry records starting at k (k is an entry), at most G
+  template <int G>
+  __device__ __forceinline__ int32_t run(int32_t k, int32_t kend) const {
+    int32_t g = 1;
+    while (g < G && k + g < kend && !(rec_w(k + g) & kMarkBit)) ++g;
+    return g;
+  }
+};
+
+// G entries of one forward column: distinct rows, all window reads first;
+// first updates start from the staged right-hand side.  Returns the number of
+// right-hand-side values consumed.
+template <int G, bool SHL, int NV>
+__device__ __forceinline__ int32_t fwd_group(Sweep<SHL, NV>& sw, int32_t k, int32_t j, int32_t v,
+                                             float val) {
+  int32_t sl[G], f[G];
+  float l[G], w[G];
+  int32_t fa = 0;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int32_t rw = sw.rec_w(k + g);
+    sl[g] = rw & kSlotMask;
+    f[g] = (rw >> 16) & 1;
+    fa |= f[g];
+    l[g] = sw.L(j + g);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * kWave];
+  int32_t used = 0;
+  if (fa) {
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      if (f[g]) {
+        w[g] = sw.V(0, v + used);
+        ++used;
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) sw.W[sl[g] * kWave] = w[g] - l[g] * val;
+  return used;
+}
+
+template <int G, bool SHL, int NV>
+__device__ __forceinline__ float bwd_group(const Sweep<SHL, NV>& sw, int32_t k, int32_t j) {
+  int32_t sl[G];
+  float l[G], w[G];
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    sl[g] = sw.rec_w(k + g) & kSlotMask;
+    l[g] = sw.L(j + g);
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * kWave];
+  float acc = 0.f;
+#pragma unroll
+  for (int g = 0; g < G; ++g) acc += l[g] * w[g];
+  return acc;
+}
+
+// Clamped sources.
+struct LaneSrc {  // ptr[i * ld + b], i clamped to [0, count)
+  gptr<const float> ptr;
+  int64_t ld, b;
+  int32_t count;
+  __device__ __forceinline__ float operator()(int32_t i) const {
+    i = i < count ? i : count - 1;
+    return (ptr + (int64_t)i * ld)[(uint32_t)b];
+  }
+};
+struct LaneSrcRev {  // ptr[(count - 1 - i) * ld + b]
+  gptr<const float> ptr;
+  int64_t ld, b;
+  int32_t count;
+  __device__ __forceinline__ float operator()(int32_t i) const {
+    i = i < count ? i : count - 1;
+    return (ptr + (int64_t)(count - 1 - i) * ld)[(uint32_t)b];
+  }
+};
+struct NoSrc {
+  __device__ __forceinline__ float operator()(int32_t) const { return 0.f; }
+};
+
+// Forward substitution, column (scatter) form, over the forward record
+// stream: a marker finalises row c (stash(c) = w[c] * D⁻¹[c]); entries apply
+// w[li[q]] -= L[q] w[c], a row's first update starting from its right-hand
+// side (rhs in first-touch order, rhs[i * ld + b]).
+// SHL: one factor shared by the batch (lx [nnz_l], dinv [n]).
 template <bool SHL>
 __device__ __forceinline__ void fwd_cols(const SweepProg& S, const float* __restrict__ lx,
                                          const float* __restrict__ dinv, int64_t f_ld, int64_t fb,
                                          const float* __restrict__ rhs, float* __restrict__ stash,
-                                         int64_t ld, int64_t b, float* lane_lds, char* wave_lds,
-                                         int lane) {
-  float* W = lane_lds;
-  float* lring = W + S.nw * kWave;
-  float* vring = lring + (SHL ? 0 : 2 * kLChunk * kWave);
-  I4* cring = reinterpret_cast<I4*>(wave_lds);
-  I2* ering = reinterpret_cast<I2*>(cring + 2 * kWave);
-  const I4* fcol = opaque(S.fcol);
-  const int32_t* fent = opaque(S.fent);
-  lx = opaque(lx);
-  dinv = opaque(dinv);
-  rhs = opaque(rhs);
-  stash = opaque(stash);
-  // column info (+ the shared dinv), entry info (+ the shared L value)
-  auto csrc = [&](int64_t c) {
-    I4 v = fcol[c];
-    if constexpr (SHL) v.z = __float_as_int(dinv[c]);
-    return v;
-  };
-  auto esrc = [&](int64_t q) {
-    I2 v;
-    v.x = fent[q];
-    v.y = SHL ? __float_as_int(lx[q]) : 0;
-    return v;
-  };
-  const LaneArray lsrc{lx, f_ld, fb};
-  const LaneArray dsrc{dinv, f_ld, fb};
-  const LaneArray rsrc{rhs, ld, b};
-  WaveStream<I4> C;
-  WaveStream<I2> E;
-  LaneStream<kLChunk> Ls;
-  LaneStream<kVChunk> Rs;
-  LaneStream<kVChunk> Ds;
-  C.init(cring, S.n, lane, csrc);
-  if (S.nnz_l > 0) E.init(ering, S.nnz_l, lane, esrc);
-  if constexpr (!SHL) {
-    if (S.nnz_l > 0) Ls.init(lring, S.nnz_l, lsrc);
-    Ds.init(vring + 2 * kVChunk * kWave, S.n, dsrc);
-  }
-  Rs.init(vring, S.n, rsrc);
-  int64_t q = 0, ir = 0;
-  for (int32_t c = 0; c < S.n; ++c) {
-    const I4 ci = C.get(c, csrc);
-    const float val = ci.x >= 0 ? W[ci.x * kWave] : Rs.get(ir++, rsrc);
-    const float di = SHL ? __int_as_float(ci.z) : Ds.get(c, dsrc);
-    (stash + (int64_t)c * ld)[(uint32_t)b] = val * di;
-    // entries of one column update distinct rows: groups of four issue all
-    // their LDS reads before any write (first updates start from the staged
-    // right-hand side instead of the window)
-    for (; q + 4 <= ci.y; q += 4) {
-      E.ensure(q + 3, esrc);
-      if constexpr (!SHL) Ls.ensure(q + 3, lsrc);
-      Rs.ensure(ir + 3, rsrc);
-      const I2 e0 = E.at(q), e1 = E.at(q + 1), e2 = E.at(q + 2), e3 = E.at(q + 3);
-      const float l0 = SHL ? __int_as_float(e0.y) : Ls.at(q);
-      const float l1 = SHL ? __int_as_float(e1.y) : Ls.at(q + 1);
-      const float l2 = SHL ? __int_as_float(e2.y) : Ls.at(q + 2);
-      const float l3 = SHL ? __int_as_float(e3.y) : Ls.at(q + 3);
-      const int32_t s0 = e0.x & (kFirstTouchBit - 1), s1 = e1.x & (kFirstTouchBit - 1);
-      const int32_t s2 = e2.x & (kFirstTouchBit - 1), s3 = e3.x & (kFirstTouchBit - 1);
-      const int32_t f0 = (e0.x >> 16) & 1, f1 = (e1.x >> 16) & 1;
-      const int32_t f2 = (e2.x >> 16) & 1, f3 = (e3.x >> 16) & 1;
-      const float w0 = W[s0 * kWave], w1 = W[s1 * kWave], w2 = W[s2 * kWave], w3 = W[s3 * kWave];
-      const float r0 = Rs.at(ir), r1 = Rs.at(ir + f0), r2 = Rs.at(ir + f0 + f1);
-      const float r3 = Rs.at(ir + f0 + f1 + f2);
-      W[s0 * kWave] = (f0 ? r0 : w0) - l0 * val;
-      W[s1 * kWave] = (f1 ? r1 : w1) - l1 * val;
-      W[s2 * kWave] = (f2 ? r2 : w2) - l2 * val;
-      W[s3 * kWave] = (f3 ? r3 : w3) - l3 * val;
-      ir += f0 + f1 + f2 + f3;
-    }
-    for (; q < ci.y; ++q) {
-      const I2 e = E.get(q, esrc);
-      const float l = SHL ? __int_as_float(e.y) : Ls.get(q, lsrc);
-      const int32_t slot = e.x & (kFirstTouchBit - 1);
-      if (e.x & kFirstTouchBit) {
-        W[slot * kWave] = Rs.get(ir++, rsrc) - l * val;
+                                         int64_t ld, int64_t b, float* lane_lds, int lane) {
+  constexpr int NV = SHL ? 1 : 2;
+  Sweep<SHL, NV> sw;
+  sw.setup(lane_lds, S.nw, lane);
+  const RecStream R = S.fwd;
+  const gptr<const I2> rec = opaque(R.rec);
+  const gptr<float> st = (gptr<float>)opaque(stash);
+  const int32_t nl = S.nnz_l > 0 ? S.nnz_l : 1;
+  const LaneSrc ls{opaque(lx), SHL ? 1 : f_ld, SHL ? 0 : fb, nl};
+  const LaneSrc ds{opaque(dinv), SHL ? 1 : f_ld, SHL ? 0 : fb, S.n};
+  const LaneSrc rs{opaque(rhs), ld, b, S.n};
+  Chunk<SHL, NV> ca, cb;
+  sw.issue(ca, rec, 0, 0, 0, 0, ls, rs, ds, ds);
+  sw.issue(cb, rec, 1, R.l1, R.v1, R.d1, ls, rs, ds, ds);
+  float val = 0.f;
+  int32_t li = 0, vi = 0, di = 0, col = 0;
+  // Each record step computes its counter increments and applies them once at
+  // the end (branch arms that end in the same increments would otherwise be
+  // merged through pointers to the counters, pinning them in scratch memory).
+  auto process = [&](int32_t lb, int32_t vb, int32_t db) __attribute__((always_inline)) {
+    const int32_t kend = 2 + sw.rec_a(1);
+    int32_t k = 2;
+    while (k < kend) {
+      const int32_t rw = sw.rec_w(k);
+      int32_t dk = 1, dl = 0, dv = 0, dd = 0;
+      if (rw & kMarkBit) {
+        const int32_t slot = rw & kSlotMask;
+        if (slot != kNoSlot) {
+          val = sw.W[slot * kWave];
+        } else {
+          val = sw.V(0, vi - vb);
+          dv = 1;
+        }
+        const float dval = sw.D(di - db);
+        dd = 1;
+        (st + (int64_t)col * ld)[(uint32_t)b] = val * dval;
+        ++col;
       } else {
-        W[slot * kWave] -= l * val;
+        const int32_t g = sw.template run<8>(k, kend);
+        if (g == 8) {
+          dv = fwd_group<8>(sw, k, li - lb, vi - vb, val);
+          dk = dl = 8;
+        } else if (g >= 4) {
+          dv = fwd_group<4>(sw, k, li - lb, vi - vb, val);
+          dk = dl = 4;
+        } else {
+          dv = fwd_group<1>(sw, k, li - lb, vi - vb, val);
+          dk = dl = 1;
+        }
       }
+      k += dk;
+      li += dl;
+      vi += dv;
+      di += dd;
     }
+  };
+  // rotated pipeline: every take-over of a chunk sits inside the loop body,
+  // where the wait for its loads is counted exactly (at a loop header the
+  // compiler would wait for all outstanding loads)
+  auto next = [&](Chunk<SHL, NV>& c, int32_t t) __attribute__((always_inline)) {
+    sw.take(c);
+    const int32_t lb = c.lbase, vb = c.vbase, db = c.dbase;
+    const int32_t l2 = sw.rec_w(0), v2 = sw.rec_a(0), d2 = sw.rec_w(1);
+    sw.issue(c, rec, t < R.nchunks ? t : R.nchunks - 1, l2, v2, d2, ls, rs, ds, ds);
+    return I4{lb, vb, db, 0};
+  };
+  I4 base = next(ca, 2);
+  for (int32_t t = 0; t < R.nchunks; t += 2) {
+    process(base.x, base.y, base.z);
+    base = next(cb, t + 3);
+    process(base.x, base.y, base.z);
+    base = next(ca, t + 4);
   }
 }
 
-// Backward substitution, column (gather) form, c = n-1 .. 0:
-//   w[c] = stash(c) - sum_q L[q] w[li[q]]   (stash already D⁻¹-scaled);
-// out(c, p[c], w[c]) receives the solution entry of original index p[c].
+// Backward substitution, column (gather) form, over the backward record
+// stream: entries accumulate sum L[q] w[li[q]]; the marker of column c sets
+// w[c] = stash(c) - sum (stash already D⁻¹-scaled) and hands it to
+// out(c, p[c], w[c]).
 template <bool SHL, class Out>
 __device__ __forceinline__ void bwd_cols(const SweepProg& S, const float* __restrict__ lx,
                                          int64_t f_ld, int64_t fb, const float* __restrict__ stash,
-                                         int64_t ld, int64_t b, float* lane_lds, char* wave_lds,
-                                         int lane, const Out& out) {
-  float* W = lane_lds;
-  float* lring = W + S.nw * kWave;
-  float* vring = lring + (SHL ? 0 : 2 * kLChunk * kWave);
-  I4* cring = reinterpret_cast<I4*>(wave_lds);
-  I2* ering = reinterpret_cast<I2*>(cring + 2 * kWave);
-  const I4* bcol = opaque(S.bcol);
-  const int32_t* fent = opaque(S.fent);
-  lx = opaque(lx);
-  stash = opaque(stash);
-  const int64_t nl = S.nnz_l;
-  auto csrc = [&](int64_t i) { return bcol[i]; };
-  // entries in descending q
-  auto esrc = [&](int64_t i) {
-    I2 v;
-    v.x = fent[nl - 1 - i];
-    v.y = SHL ? __float_as_int(lx[nl - 1 - i]) : 0;
-    return v;
+                                         int64_t ld, int64_t b, float* lane_lds, int lane,
+                                         const Out& out) {
+  Sweep<SHL, 1> sw;
+  sw.setup(lane_lds, S.nw, lane);
+  const RecStream R = S.bwd;
+  const gptr<const I2> rec = opaque(R.rec);
+  const int32_t nl = S.nnz_l > 0 ? S.nnz_l : 1;
+  const LaneSrcRev ls{opaque(lx), SHL ? 1 : f_ld, SHL ? 0 : fb, nl};
+  const LaneSrcRev ss{opaque(stash), ld, b, S.n};
+  const NoSrc none{};
+  Chunk<SHL, 1> ca, cb;
+  sw.issue(ca, rec, 0, 0, 0, 0, ls, ss, none, none);
+  sw.issue(cb, rec, 1, R.l1, R.v1, R.d1, ls, ss, none, none);
+  float corr = 0.f;
+  int32_t li = 0, vi = 0, col = S.n - 1;
+  auto process = [&](int32_t lb, int32_t vb) __attribute__((always_inline)) {
+    const int32_t kend = 2 + sw.rec_a(1);
+    int32_t k = 2;
+    while (k < kend) {
+      const int32_t rw = sw.rec_w(k);
+      int32_t dk = 1, dl = 0, dv = 0;
+      if (rw & kMarkBit) {
+        const float w = sw.V(0, vi - vb) - corr;
+        dv = 1;
+        corr = 0.f;
+        const int32_t slot = rw & kSlotMask;
+        if (slot != kNoSlot) sw.W[slot * kWave] = w;
+        out(col, sw.rec_a(k), w);
+        --col;
+      } else {
+        const int32_t g = sw.template run<8>(k, kend);
+        if (g == 8) {
+          corr += bwd_group<8>(sw, k, li - lb);
+          dk = dl = 8;
+        } else if (g >= 4) {
+          corr += bwd_group<4>(sw, k, li - lb);
+          dk = dl = 4;
+        } else {
+          corr += bwd_group<1>(sw, k, li - lb);
+          dk = dl = 1;
+        }
+      }
+      k += dk;
+      li += dl;
+      vi += dv;
+    }
   };
-  const LaneArrayRev lsrc{lx, f_ld, fb, nl};
-  const LaneArrayRev ssrc{stash, ld, b, S.n};
-  WaveStream<I4> C;
-  WaveStream<I2> E;
-  LaneStream<kLChunk> Ls;
-  LaneStream<kVChunk> Ss;
-  C.init(cring, S.n, lane, csrc);
-  if (nl > 0) E.init(ering, nl, lane, esrc);
-  if constexpr (!SHL) {
-    if (nl > 0) Ls.init(lring, nl, lsrc);
-  }
-  Ss.init(vring, S.n, ssrc);
-  int64_t i = 0;  // entries consumed (q = nl - 1 - i)
-  for (int32_t k = 0; k < S.n; ++k) {
-    const I4 ci = C.get(k, csrc);  // column c = n-1-k
-    float corr = 0.f;
-    // groups of four: independent window reads, one accumulation chain
-    for (; nl - 4 - i >= ci.y; i += 4) {
-      E.ensure(i + 3, esrc);
-      if constexpr (!SHL) Ls.ensure(i + 3, lsrc);
-      const I2 e0 = E.at(i), e1 = E.at(i + 1), e2 = E.at(i + 2), e3 = E.at(i + 3);
-      const float l0 = SHL ? __int_as_float(e0.y) : Ls.at(i);
-      const float l1 = SHL ? __int_as_float(e1.y) : Ls.at(i + 1);
-      const float l2 = SHL ? __int_as_float(e2.y) : Ls.at(i + 2);
-      const float l3 = SHL ? __int_as_float(e3.y) : Ls.at(i + 3);
-      const float w0 = W[(e0.x & (kFirstTouchBit - 1)) * kWave];
-      const float w1 = W[(e1.x & (kFirstTouchBit - 1)) * kWave];
-      const float w2 = W[(e2.x & (kFirstTouchBit - 1)) * kWave];
-      const float w3 = W[(e3.x & (kFirstTouchBit - 1)) * kWave];
-      corr += l0 * w0;
-      corr += l1 * w1;
-      corr += l2 * w2;
-      corr += l3 * w3;
-    }
-    for (; nl - 1 - i >= ci.y; ++i) {
-      const I2 e = E.get(i, esrc);
-      const float l = SHL ? __int_as_float(e.y) : Ls.get(i, lsrc);
-      corr += l * W[(e.x & (kFirstTouchBit - 1)) * kWave];
-    }
-    const float w = Ss.get(k, ssrc) - corr;
-    if (ci.x >= 0) W[ci.x * kWave] = w;
-    out(S.n - 1 - k, ci.z, w);
+  auto next = [&](Chunk<SHL, 1>& c, int32_t t) __attribute__((always_inline)) {
+    sw.take(c);
+    const int32_t lb = c.lbase, vb = c.vbase;
+    const int32_t l2 = sw.rec_w(0), v2 = sw.rec_a(0);
+    sw.issue(c, rec, t < R.nchunks ? t : R.nchunks - 1, l2, v2, 0, ls, ss, none, none);
+    return I2{lb, vb};
+  };
+  I2 base = next(ca, 2);
+  for (int32_t t = 0; t < R.nchunks; t += 2) {
+    process(base.x, base.y);
+    base = next(cb, t + 3);
+    process(base.x, base.y);
+    base = next(ca, t + 4);
   }
 }
 
