@@ -83,6 +83,23 @@ class _OsqpDesc(ctypes.Structure):
     ]
 
 
+class _AdjDesc(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("m", ctypes.c_int32), ("n_l", ctypes.c_int32), ("n_u", ctypes.c_int32),
+        ("n_eq", ctypes.c_int32), ("dim", ctypes.c_int32), ("nnz_adj", ctypes.c_int32),
+        ("nnz_p", ctypes.c_int32), ("nnz_a", ctypes.c_int32), ("refine", ctypes.c_int32),
+        ("eps", ctypes.c_double),
+        ("kkt", ctypes.POINTER(_Pattern)),
+        ("adj_rows", _i32p), ("adj_cols", _i32p), ("adj_base", _f64p),
+        ("lower", _i32p), ("upper", _i32p), ("eq", _i32p),
+        ("n_pf", ctypes.c_int32), ("n_g", ctypes.c_int32), ("n_e", ctypes.c_int32),
+        ("p_full_map", _i32p), ("g_rows", _i32p), ("g_cols", _i32p), ("g_map", _i32p), ("g_sign", _f64p),
+        ("e_map", _i32p), ("diag_first_pos", _i32p), ("diag_second_pos", _i32p), ("p_pos", _i32p),
+        ("g_pos", _i32p), ("aeq_pos", _i32p), ("glambda_pos", _i32p), ("slack_pos", _i32p),
+        ("aeq_t_pos", _i32p), ("p_rows", _i32p), ("p_cols", _i32p), ("a_rows", _i32p), ("a_cols", _i32p),
+    ]
+
+
 def build() -> pathlib.Path:
     """Compile ``liboracle.so`` with the committed Makefile."""
     subprocess.run(["make", "-s", "-C", str(_HERE)], check=True)
@@ -104,6 +121,9 @@ def lib() -> ctypes.CDLL:
                 ctypes.POINTER(_Pattern), i64, vp, i64, vp, i64, vp, vp]
             getattr(_LIB, f"oracle_osqp_solve_{sfx}").argtypes = [
                 ctypes.POINTER(_OsqpDesc), i64, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+            getattr(_LIB, f"oracle_adjoint_{sfx}").argtypes = [
+                ctypes.POINTER(_AdjDesc), i64, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                vp, vp]
         _LIB.oracle_num_threads.restype = ctypes.c_int
     return _LIB
 
@@ -258,6 +278,74 @@ class OracleOSQP:
         return x, z, y, prim, dual, obj
 
 
+class OracleAdjoint:
+    """OSQP adjoint derivative oracle bound to a plan built with ``derivatives=True``.
+
+    ``compute`` returns ``(sol, ryl, ryu, dq, dl, du, dp, da)`` per instance,
+    restating ``solver.py:1035-1192``; ``refine`` is
+    ``derivative_refinement_iters``.
+    """
+
+    def __init__(self, plan, refine: int = 100):
+        dp = plan.derivative_plan
+        if dp is None:
+            raise ValueError("plan has no derivative plan (build_osqp_plan(..., derivatives=True))")
+        self.plan = plan
+        self.dp = dp
+        self.pattern = OraclePattern(dp.qdldl_plan)
+        i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        adj = dp.adjoint_upper
+        rows = np.asarray(adj.indices, dtype=np.int32)
+        cols = np.repeat(np.arange(dp.adjoint_dim, dtype=np.int32), np.diff(adj.indptr))
+        self.keep = {
+            "adj_rows": i32(rows), "adj_cols": i32(cols), "adj_base": f64(adj.data),
+            "lower": i32(dp.lower_indices), "upper": i32(dp.upper_indices), "eq": i32(dp.eq_indices),
+            "p_full_map": i32(dp.p_full_value_map), "g_rows": i32(dp.g_rows), "g_cols": i32(dp.g_cols),
+            "g_map": i32(dp.g_a_value_map), "g_sign": f64(dp.g_sign), "e_map": i32(dp.aeq_a_value_map),
+            "diag_first_pos": i32(dp.diag_first_pos), "diag_second_pos": i32(dp.diag_second_pos),
+            "p_pos": i32(dp.p_block_pos), "g_pos": i32(dp.g_block_pos), "aeq_pos": i32(dp.aeq_block_pos),
+            "glambda_pos": i32(dp.glambda_block_pos), "slack_pos": i32(dp.slack_diag_pos),
+            "aeq_t_pos": i32(dp.aeq_t_block_pos), "p_rows": i32(dp.p_upper_rows), "p_cols": i32(dp.p_upper_cols),
+            "a_rows": i32(plan.a_rows), "a_cols": i32(plan.a_cols),
+        }
+        d = _AdjDesc()
+        d.n, d.m = dp.n, dp.m
+        d.n_l, d.n_u, d.n_eq = dp.n_ineq_l, dp.n_ineq_u, dp.n_eq
+        d.dim, d.nnz_adj = dp.adjoint_dim, adj.nnz
+        d.nnz_p, d.nnz_a = plan.nnz_p, plan.nnz_a
+        d.refine = int(refine)
+        d.eps = dp.eps
+        d.kkt = ctypes.pointer(self.pattern.struct)
+        d.n_pf, d.n_g, d.n_e = dp.p_full_value_map.size, dp.g_rows.size, dp.aeq_rows.size
+        for name, arr in self.keep.items():
+            setattr(d, name, arr.ctypes.data_as(_f64p if arr.dtype == np.float64 else _i32p))
+        self.desc = d
+
+    def compute(self, p_values, a_values, l, u, x, y, dx, dy, dtype=np.float64):
+        sfx = _dtype_sfx(dtype)
+        plan = self.plan
+        n, m = plan.n, plan.m
+        arr = lambda v, w: np.ascontiguousarray(np.atleast_2d(np.asarray(v, dtype=dtype)))  # noqa: E731
+        pv, av = arr(p_values, plan.nnz_p), arr(a_values, plan.nnz_a)
+        ins = [arr(v, 0) for v in (l, u, x, y, dx, dy)]
+        b = max(pv.shape[0], av.shape[0], *(v.shape[0] for v in ins))
+        widths = (m, m, n, m, n, m)
+        ins = [np.ascontiguousarray(np.broadcast_to(v, (b, w))) for v, w in zip(ins, widths)]
+        ps = 0 if pv.shape[0] == 1 else plan.nnz_p
+        as_ = 0 if av.shape[0] == 1 else plan.nnz_a
+        if ps:
+            pv = np.ascontiguousarray(np.broadcast_to(pv, (b, plan.nnz_p)))
+        if as_:
+            av = np.ascontiguousarray(np.broadcast_to(av, (b, plan.nnz_a)))
+        out = [np.zeros((b, w), dtype=dtype) for w in (self.dp.adjoint_dim, m, m, n, m, m, plan.nnz_p, plan.nnz_a)]
+        rc = getattr(lib(), f"oracle_adjoint_{sfx}")(
+            ctypes.byref(self.desc), b, _ptr(pv), ps, _ptr(av), as_, *(_ptr(v) for v in ins), *(_ptr(o) for o in out))
+        if rc:
+            raise RuntimeError("oracle adjoint failed")
+        return tuple(out)
+
+
 def num_threads() -> int:
     return int(lib().oracle_num_threads())
 
@@ -266,4 +354,5 @@ def set_threads(n: int) -> None:
     os.environ["OMP_NUM_THREADS"] = str(int(n))
 
 
-__all__ = ["OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor", "qdldl_solve"]
+__all__ = ["OracleAdjoint", "OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor",
+           "qdldl_solve"]

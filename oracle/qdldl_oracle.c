@@ -15,7 +15,9 @@
  *     x *= Dinv, backward column gather, out = x[pinv];
  *   - the fixed-iteration OSQP ADMM of warpmpc/jax_osqp/solver.py:764-872:
  *     KKT value scatter, input scaling, max_iter iterations of _iterate,
- *     unscaling, residuals and objective.
+ *     unscaling, residuals and objective;
+ *   - OSQP's adjoint derivative of warpmpc/jax_osqp/solver.py:1035-1192
+ *     (adjoint_oracle_impl.h).
  *
  * Compiled twice from qdldl_oracle_impl.h: suffix _f64 (REAL = double, the
  * oracle proper) and _f32 (REAL = float, the FP32 CPU counterpart of the
@@ -31,12 +33,14 @@
 #define REAL double
 #define SFX(name) name##_f64
 #include "qdldl_oracle_impl.h"
+#include "adjoint_oracle_impl.h"
 #undef REAL
 #undef SFX
 
 #define REAL float
 #define SFX(name) name##_f32
 #include "qdldl_oracle_impl.h"
+#include "adjoint_oracle_impl.h"
 #undef REAL
 #undef SFX
 

@@ -129,3 +129,23 @@ def test_converged_oracle_satisfies_qp_optimality():
     assert np.all((y <= 1e-8) | (np.abs(ax - u) < 1e-7))
     assert np.all((y >= -1e-8) | (np.abs(ax - l) < 1e-7))
     np.testing.assert_allclose(p.toarray() @ x + q + a.toarray().T @ y, 0.0, atol=1e-8)
+
+
+def test_nan_input_propagates_and_stays_per_instance():
+    """A NaN in one instance's q reaches its outputs (the box projection and the
+    residual maxima propagate NaN instead of clamping it away) and leaves the
+    other instances untouched."""
+    p = sp.csc_matrix([[4.0, 1.0], [1.0, 2.0]])
+    a = sp.csc_matrix([[1.0, 1.0], [1.0, 0.0], [0.0, 1.0]])
+    l = np.array([1.0, 0.0, 0.0])
+    u = np.array([1.0, 0.7, 0.7])
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=50)
+    plan = build_osqp_plan(sp.triu(p, format="csc"), a, l, u, s)
+    orc = oracle.OracleOSQP(plan)
+    q = np.array([[1.0, 1.0], [np.nan, 1.0], [1.0, 1.0]])
+    x, z, y, prim, dual, obj = orc.solve(plan.p_upper.data[None], plan.a_matrix.data[None], q, l, u)
+    assert np.isnan(x[1]).all() and np.isnan(z[1]).all() and np.isnan(y[1]).all()
+    assert np.isnan(prim[1]) and np.isnan(dual[1]) and np.isnan(obj[1])
+    for i in (0, 2):
+        assert np.isfinite(x[i]).all() and np.isfinite(prim[i])
+    np.testing.assert_array_equal(x[0], x[2])
