@@ -188,6 +188,79 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         int64_t work_len, float* x, float* z, float* y, float* prim, float* dual,
                         float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream);
 
+/*
+ * OSQP adjoint derivative (warpmpc/jax_osqp/solver.py:1035-1192).  The
+ * descriptor carries the OSQPDerivativePlan fields (types.py:21-55, built by
+ * solver.py:413-571) and the A entry coordinates of the FixedOSQPPlan.  With
+ * half = n + n_l + n_u + n_eq the adjoint matrix is 2*half square, stored as
+ * its upper triangle in CSC form.  Host pointers; read during plan creation.
+ */
+typedef struct wmpc_adjoint_desc {
+  int32_t n, m, n_l, n_u, n_eq;
+  int32_t nnz_adj, nnz_p, nnz_a;
+  int32_t n_p_full, n_g, n_aeq;
+  double eps;
+  const int32_t* adj_indptr;        /* [2*half + 1] adjoint_upper */
+  const int32_t* adj_indices;       /* [nnz_adj] */
+  const double* adj_data;           /* [nnz_adj] structural values */
+  const int32_t* lower_indices;     /* [n_l] */
+  const int32_t* upper_indices;     /* [n_u] */
+  const int32_t* eq_indices;        /* [n_eq] */
+  const int32_t* p_upper_rows;      /* [nnz_p] */
+  const int32_t* p_upper_cols;      /* [nnz_p] */
+  const int32_t* p_full_value_map;  /* [n_p_full] */
+  const int32_t* g_rows;            /* [n_g] inequality row of each G entry */
+  const int32_t* g_cols;            /* [n_g] */
+  const int32_t* g_a_value_map;     /* [n_g] */
+  const double* g_sign;             /* [n_g] */
+  const int32_t* aeq_a_value_map;   /* [n_aeq] */
+  const int32_t* diag_first_pos;    /* [half] */
+  const int32_t* diag_second_pos;   /* [half] */
+  const int32_t* p_block_pos;       /* [n_p_full] */
+  const int32_t* g_block_pos;       /* [n_g] */
+  const int32_t* aeq_block_pos;     /* [n_aeq] */
+  const int32_t* glambda_block_pos; /* [n_g] */
+  const int32_t* slack_diag_pos;    /* [n_l + n_u] */
+  const int32_t* aeq_t_block_pos;   /* [n_aeq] */
+  const int32_t* a_rows;            /* [nnz_a] */
+  const int32_t* a_cols;            /* [nnz_a] */
+} wmpc_adjoint_desc;
+
+typedef struct wmpc_adjoint_plan wmpc_adjoint_plan;
+
+/* Replaces the constant setup of the adjoint in compile_osqp
+ * (solver.py:984-1029).  `adj` must be the QDLDL plan of desc's adjoint
+ * pattern (OSQPDerivativePlan.qdldl_plan); it is referenced, not copied, and
+ * must outlive the adjoint plan. */
+int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* desc, const wmpc_qdldl_plan* adj,
+                             wmpc_adjoint_plan** out);
+int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan);
+/* Adjoint dimension 2*half (the width of `sol`). */
+int32_t wmpc_adjoint_dim(const wmpc_adjoint_plan* plan);
+/* Floats of scratch wmpc_osqp_adjoint_f32 needs at instance stride ld:
+ * (nnz_l + 4*dim + 2*(n_l + n_u)) * ld.  -1 on bad input. */
+int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld);
+
+/*
+ * adjoint_derivative_compute + get_vec + get_mat (solver.py:1047-1192) in one
+ * launch, one lane per instance: multipliers and slacks from (x, y), the
+ * eps-regularised adjoint matrix assembled and factored, the solve, `refine`
+ * refinement steps against the unperturbed matrix, then
+ *   dq = rx [n][ld], dl = ryl, du = -ryu [m][ld],
+ *   dp [nnz_p][ld] (may be NULL), da [nnz_a][ld] (may be NULL),
+ *   sol [dim][ld] (may be NULL), the refined adjoint solution.
+ * p_values/a_values: [nnz][pm_ld], pm_ld = 0 when shared by the batch.
+ * l, u, y, dy [m][ld]; x, dx [n][ld].  dy is the cotangent of the split
+ * multipliers (y_l, y_u), i.e. of |y|, as in the reference.
+ * work: work_len >= wmpc_adjoint_work_floats(plan, ld) floats.
+ */
+int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
+                          const float* a_values, int64_t pm_ld, const float* l, const float* u,
+                          const float* x, const float* y, const float* dx, const float* dy,
+                          int32_t refine, float* work, int64_t work_len, float* sol, float* dq,
+                          float* dl, float* du, float* dp, float* da, int64_t batch, int64_t ld,
+                          void* stream);
+
 #ifdef __cplusplus
 }
 #endif

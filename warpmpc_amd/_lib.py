@@ -87,6 +87,49 @@ class OsqpDesc(ctypes.Structure):
     ]
 
 
+class AdjointDesc(ctypes.Structure):
+    """Mirror of ``wmpc_adjoint_desc``."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("n_l", ctypes.c_int32),
+        ("n_u", ctypes.c_int32),
+        ("n_eq", ctypes.c_int32),
+        ("nnz_adj", ctypes.c_int32),
+        ("nnz_p", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("n_p_full", ctypes.c_int32),
+        ("n_g", ctypes.c_int32),
+        ("n_aeq", ctypes.c_int32),
+        ("eps", ctypes.c_double),
+        ("adj_indptr", _i32p),
+        ("adj_indices", _i32p),
+        ("adj_data", _f64p),
+        ("lower_indices", _i32p),
+        ("upper_indices", _i32p),
+        ("eq_indices", _i32p),
+        ("p_upper_rows", _i32p),
+        ("p_upper_cols", _i32p),
+        ("p_full_value_map", _i32p),
+        ("g_rows", _i32p),
+        ("g_cols", _i32p),
+        ("g_a_value_map", _i32p),
+        ("g_sign", _f64p),
+        ("aeq_a_value_map", _i32p),
+        ("diag_first_pos", _i32p),
+        ("diag_second_pos", _i32p),
+        ("p_block_pos", _i32p),
+        ("g_block_pos", _i32p),
+        ("aeq_block_pos", _i32p),
+        ("glambda_block_pos", _i32p),
+        ("slack_diag_pos", _i32p),
+        ("aeq_t_block_pos", _i32p),
+        ("a_rows", _i32p),
+        ("a_cols", _i32p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/warpmpc_hip.h declares
 SIGNATURES: dict[str, tuple] = {
     "wmpc_last_error": (ctypes.c_char_p, []),
@@ -109,6 +152,13 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_osqp_solve_f32": (ctypes.c_int, [
         _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
+    "wmpc_adjoint_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointDesc), _vp, ctypes.POINTER(_vp)]),
+    "wmpc_adjoint_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_adjoint_dim": (ctypes.c_int32, [_vp]),
+    "wmpc_adjoint_work_floats": (ctypes.c_int64, [_vp, _i64]),
+    "wmpc_osqp_adjoint_f32": (ctypes.c_int, [
+        _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i64,
+        _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
 }
 
 PROGRAM_TABLES = ("row", "init", "proc", "upd", "bcol", "bent", "fcol", "bcol2", "fent", "ftpos", "frec",
@@ -174,5 +224,5 @@ def f64p(a: np.ndarray):
     return a.ctypes.data_as(_f64p)
 
 
-__all__ = ["HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
+__all__ = ["AdjointDesc", "HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
            "load", "plan_program", "plan_slots"]

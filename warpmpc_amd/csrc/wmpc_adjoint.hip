@@ -1,0 +1,459 @@
+// OSQP adjoint derivative for MI355X (gfx950) behind include/warpmpc_hip.h.
+//
+// One launch, one lane per instance, restating warpmpc/jax_osqp/solver.py:
+// 1047-1192.  With G = [-A_lower; A_upper], h = [-l_lower; u_upper], the split
+// multipliers lambda = [max(-y, 0)_lower; max(y, 0)_upper] and the slacks
+// s = G x - h, the adjoint matrix is
+//     M = [[(1 + eps) I, U], [Uᵀ, -eps I]],
+//     U = [[P, Gᵀ diag(lambda), Aeqᵀ], [G, diag(s), 0], [Aeq, 0, 0]],
+// and M0 the same with eps = 0.  Per instance the kernel
+//   1. evaluates lambda and s (G x by rows of G) into the work area;
+//   2. factors M with the up-looking LDLᵀ of wmpc_qdldl_device.h, its values
+//      evaluated slot by slot from (P, A, lambda, s), the forward
+//      substitution of rhs = [-dx; -dy_lower; -dy_upper; -d_nu; 0] fused in;
+//   3. runs the backward sweep into sol, then `refine` steps
+//      sol -= M⁻¹ (M0 sol - rhs), M0 sol by rows of the full symmetric M0;
+//   4. maps sol back to dq, dl, du and forms dP, dA.
+// Every table the kernel reads is wave-uniform (scalar loads); every
+// per-instance array is batch-major [entry][ld].
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "wmpc_blob.h"
+#include "wmpc_plan.h"
+#include "wmpc_stream.h"
+
+namespace wmpc {
+
+// Value sources of an adjoint slot {kind, i0, i1, f bits}:
+enum : int32_t {
+  kSlotConst = 0,    // f; i0 holds the bits of the perturbed value
+  kSlotP = 1,        // p[i0]
+  kSlotA = 2,        // a[i0] * f
+  kSlotGLambda = 3,  // lambda[i1] * (a[i0] * f)
+  kSlotSlack = 4,    // s[i0]
+};
+// Right-hand-side sources {kind, index} of adjoint row o.
+enum : int32_t { kRhsDx = 0, kRhsDy = 1, kRhsEq = 2, kRhsZero = 3 };
+
+struct AdjDev {
+  int32_t n = 0, m = 0, ni = 0, half = 0, dim = 0, nnz_p = 0, nnz_a = 0;
+  const int4* slot = nullptr;      // [nnz_adj]
+  const int32_t* mv_ptr = nullptr; // [dim + 1] rows of the full symmetric M0
+  const int2* mv = nullptr;        // {col, slot}
+  const int32_t* g_ptr = nullptr;  // [ni + 1] rows of G
+  const int4* g = nullptr;         // {col, A value index, sign bits, 0}
+  const int2* ineq = nullptr;      // [ni] {constraint index, 1 if upper-bounded}
+  const int2* rsrc = nullptr;      // [dim] right-hand-side source
+  const int4* con = nullptr;       // [m] {lower, upper, equality position in sol or -1, 0}
+  const int2* pent = nullptr;      // [nnz_p] {row, col}
+  const int2* aent = nullptr;      // [nnz_a] {row, col}
+};
+
+}  // namespace wmpc
+
+struct wmpc_adjoint_plan {
+  wmpc::AdjDev d;
+  const wmpc_qdldl_plan* adj = nullptr;
+  void* block = nullptr;
+};
+
+namespace wmpc {
+
+// Adjoint values of one instance.  PERT: the factored matrix M (eps on both
+// diagonals), else M0.  P/A are [nnz][pm_ld] per instance or shared columns.
+template <bool PERT>
+struct AdjSource {
+  const int4* __restrict__ slot;
+  const float* __restrict__ p;
+  const float* __restrict__ a;
+  int64_t pes;   // entry stride of P/A
+  uint32_t pl;   // lane offset into P/A (0 when shared)
+  CStrided lam, slk;
+  __device__ __forceinline__ float operator()(int32_t s, int64_t = 0) const {
+    const int4 t = slot[s];
+    switch (t.x) {
+      case kSlotConst:
+        return __int_as_float(PERT ? t.y : t.w);
+      case kSlotP:
+        return (p + (int64_t)t.y * pes)[pl];
+      case kSlotA:
+        return (a + (int64_t)t.y * pes)[pl] * __int_as_float(t.w);
+      case kSlotGLambda:
+        return lam(t.z) * ((a + (int64_t)t.y * pes)[pl] * __int_as_float(t.w));
+      default:
+        return slk(t.y);
+    }
+  }
+};
+
+// Right-hand side of adjoint row o (solver.py:1118-1124).
+struct AdjRhs {
+  const int2* __restrict__ rsrc;
+  CStrided dx, dy, y;
+  __device__ __forceinline__ float at(int32_t o) const {
+    const int2 r = rsrc[o];
+    if (r.x == kRhsDx) return -dx(r.y);
+    if (r.x == kRhsDy) return -dy(r.y);
+    if (r.x == kRhsEq) {
+      const float d = dy(r.y);
+      return -(y(r.y) >= 0.0f ? d : -d);
+    }
+    return 0.0f;
+  }
+  __device__ __forceinline__ float operator()(int32_t, int32_t o) const { return at(o); }
+};
+
+struct SolSet {
+  Strided sol;
+  __device__ __forceinline__ void operator()(int32_t, int32_t o, float v) const { sol(o) = v; }
+};
+struct SolSub {
+  Strided sol;
+  __device__ __forceinline__ void operator()(int32_t, int32_t o, float v) const {
+    sol(o) = sol(o) - v;
+  }
+};
+
+__global__ __launch_bounds__(kWave) void k_osqp_adjoint(
+    QdldlProg P, AdjDev dv, const float* __restrict__ pv, const float* __restrict__ av,
+    int64_t pm_ld, const float* __restrict__ l, const float* __restrict__ u,
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ dx,
+    const float* __restrict__ dy, float* __restrict__ lx, float* __restrict__ dinv,
+    float* __restrict__ stash, float* __restrict__ rbuf, float* __restrict__ sol,
+    float* __restrict__ lam, float* __restrict__ slk, float* __restrict__ dq,
+    float* __restrict__ dl, float* __restrict__ du, float* __restrict__ dp,
+    float* __restrict__ da, int32_t refine, int64_t batch, int64_t ld) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
+  // lanes past the batch mirror the last instance so that every lane takes
+  // part in the wave-shared staging of the sweeps
+  const int64_t b = b0 < batch ? b0 : batch - 1;
+  float* mylds = lds + lane;
+  const int64_t pes = pm_ld ? pm_ld : 1;
+  const uint32_t pl = pm_ld ? (uint32_t)b : 0u;
+  const CStrided X{x, ld, b}, Y{y, ld, b}, Lo{l, ld, b}, Up{u, ld, b};
+  const Strided LAM{lam, ld, b}, SLK{slk, ld, b}, SOL{sol, ld, b}, RB{rbuf, ld, b};
+  // 1. multipliers and slacks of the inequality rows (solver.py:1082-1097)
+  for (int32_t i = 0; i < dv.ni; ++i) {
+    const int2 q = dv.ineq[i];
+    const float yv = Y(q.x);
+    float s = 0.0f;
+    for (int32_t e = dv.g_ptr[i]; e < dv.g_ptr[i + 1]; ++e) {
+      const int4 t = dv.g[e];
+      const float gv = (av + (int64_t)t.y * pes)[pl] * __int_as_float(t.z);
+      s += gv * X(t.x);
+    }
+    LAM(i) = q.y ? nan_max(yv, 0.0f) : -nan_min(yv, 0.0f);
+    SLK(i) = s - (q.y ? Up(q.x) : -Lo(q.x));
+  }
+  const CStrided CLAM{lam, ld, b}, CSLK{slk, ld, b};
+  const AdjSource<true> msrc{dv.slot, pv, av, pes, pl, CLAM, CSLK};
+  const AdjSource<false> m0{dv.slot, pv, av, pes, pl, CLAM, CSLK};
+  const AdjRhs rhs{dv.rsrc, CStrided{dx, ld, b}, CStrided{dy, ld, b}, Y};
+  // 2. factor M with the forward substitution of rhs fused in
+  factor_rows<true>(P, msrc, 0, Strided{lx, ld, b}, Strided{nullptr, 0, 0}, Strided{dinv, ld, b},
+                    mylds, rhs, Strided{stash, ld, b});
+  // 3. backward sweep, then the refinement against M0 (solver.py:1127-1139)
+  const SweepProg SP = sweep_prog(P);
+  bwd_cols<false>(SP, lx, ld, b, stash, ld, b, mylds, lane, SolSet{SOL});
+  for (int32_t it = 0; it < refine; ++it) {
+    // residual M0 sol - rhs, staged in the sweep's first-touch order
+    for (int32_t i = 0; i < dv.dim; ++i) {
+      const int32_t o = P.rperm[i];
+      float s = 0.0f;
+      for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
+        const int2 t = dv.mv[e];
+        s += m0(t.y) * SOL(t.x);
+      }
+      RB(i) = s - rhs.at(o);
+    }
+    fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, mylds, lane);
+    bwd_cols<false>(SP, lx, ld, b, stash, ld, b, mylds, lane, SolSub{SOL});
+  }
+  // 4. outputs (solver.py:1141-1192); only real lanes store
+  if (b0 >= batch) return;
+  const Strided DQ{dq, ld, b}, DL{dl, ld, b}, DU{du, ld, b};
+  const int32_t n = dv.n, half = dv.half;
+  for (int32_t i = 0; i < n; ++i) DQ(i) = SOL(half + i);
+  for (int32_t i = 0; i < dv.m; ++i) {
+    const int4 t = dv.con[i];
+    const float yv = Y(i);
+    const float yu = nan_max(yv, 0.0f), yl = -nan_min(yv, 0.0f);
+    float rl = t.x >= 0 ? -SOL(t.x) : 0.0f;
+    float ru = t.y >= 0 ? SOL(t.y) : 0.0f;
+    if (t.z >= 0) {
+      const float es = SOL(t.z);
+      const bool pos = yv >= 0.0f;
+      ru = pos ? es / yv : 0.0f;
+      rl = pos ? 0.0f : -es / yv;
+    }
+    DL(i) = -(rl * yl);
+    DU(i) = -(ru * yu);
+  }
+  if (dp) {
+    const Strided DP{dp, ld, b};
+    for (int32_t k = 0; k < dv.nnz_p; ++k) {
+      const int2 t = dv.pent[k];
+      DP(k) = 0.5f * (SOL(half + t.x) * X(t.y) + SOL(half + t.y) * X(t.x));
+    }
+  }
+  if (da) {
+    // (y_u - y_l) = y and (ryu - ryl) = -du - dl, exactly
+    const Strided DA{da, ld, b};
+    for (int32_t k = 0; k < dv.nnz_a; ++k) {
+      const int2 t = dv.aent[k];
+      DA(k) = Y(t.x) * SOL(half + t.y) + (-DU(t.x) - DL(t.x)) * X(t.y);
+    }
+  }
+}
+
+static int32_t adjoint_lane_slots(const QdldlProg& P) {
+  return std::max(factor_lds_slots(P, true), sweep_lane_slots(sweep_prog(P), false));
+}
+
+}  // namespace wmpc
+
+using namespace wmpc;
+
+extern "C" {
+
+int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* ds, const wmpc_qdldl_plan* adj,
+                             wmpc_adjoint_plan** out) {
+  clear_error();
+  WMPC_REQUIRE(ds && adj && out, "null argument");
+  *out = nullptr;
+  WMPC_REQUIRE(adj->block, "host-only adjoint plan: create it with wmpc_qdldl_plan_create");
+  const int32_t n = ds->n, m = ds->m, nl = ds->n_l, nu = ds->n_u, ne = ds->n_eq;
+  WMPC_REQUIRE(n > 0 && m >= 0 && nl >= 0 && nu >= 0 && ne >= 0 && nl <= m && nu <= m && ne <= m,
+               "bad dimensions");
+  const int32_t ni = nl + nu, half = n + ni + ne, dim = 2 * half, nz = ds->nnz_adj;
+  WMPC_REQUIRE(adj->host.n == dim, "adjoint QDLDL plan dimension must be 2*(n + n_l + n_u + n_eq)");
+  WMPC_REQUIRE(adj->host.nnz_a == nz, "adjoint nonzero count differs from the QDLDL plan");
+  WMPC_REQUIRE(ds->nnz_p >= 0 && ds->nnz_a >= 0 && ds->n_p_full >= 0 && ds->n_g >= 0 && ds->n_aeq >= 0,
+               "bad counts");
+  WMPC_REQUIRE(ds->adj_indptr && ds->adj_indices && ds->adj_data && ds->diag_first_pos &&
+                   ds->diag_second_pos && (nl == 0 || ds->lower_indices) &&
+                   (nu == 0 || ds->upper_indices) && (ne == 0 || ds->eq_indices) &&
+                   (ds->nnz_p == 0 || (ds->p_upper_rows && ds->p_upper_cols)) &&
+                   (ds->n_p_full == 0 || (ds->p_full_value_map && ds->p_block_pos)) &&
+                   (ds->n_g == 0 || (ds->g_rows && ds->g_cols && ds->g_a_value_map && ds->g_sign &&
+                                     ds->g_block_pos && ds->glambda_block_pos)) &&
+                   (ds->n_aeq == 0 || (ds->aeq_a_value_map && ds->aeq_block_pos && ds->aeq_t_block_pos)) &&
+                   (ni == 0 || ds->slack_diag_pos) && (ds->nnz_a == 0 || (ds->a_rows && ds->a_cols)),
+               "descriptor has a null array");
+  // CSC structure of the upper triangle
+  WMPC_REQUIRE(ds->adj_indptr[0] == 0 && ds->adj_indptr[dim] == nz, "adj_indptr does not span nnz_adj");
+  std::vector<int32_t> rows(nz), cols(nz);
+  for (int32_t c = 0; c < dim; ++c) {
+    WMPC_REQUIRE(ds->adj_indptr[c] <= ds->adj_indptr[c + 1], "adj_indptr not monotone");
+    for (int32_t k = ds->adj_indptr[c]; k < ds->adj_indptr[c + 1]; ++k) {
+      const int32_t r = ds->adj_indices[k];
+      WMPC_REQUIRE(r >= 0 && r <= c, "adjoint entry outside the upper triangle");
+      rows[k] = r;
+      cols[k] = c;
+    }
+  }
+  // slot sources
+  std::vector<int32_t> slot(4 * (size_t)nz, 0);
+  std::vector<char> set(nz, 0);
+  for (int32_t k = 0; k < nz; ++k) {
+    slot[4 * k] = kSlotConst;
+    slot[4 * k + 1] = float_bits(ds->adj_data[k]);
+    slot[4 * k + 3] = float_bits(ds->adj_data[k]);
+  }
+  std::string err;
+  auto claim = [&](int32_t pos, int32_t kind, int32_t i0, int32_t i1, double f) {
+    if (pos < 0 || pos >= nz || set[pos]) {
+      err = "adjoint block position out of range or repeated";
+      return;
+    }
+    set[pos] = 1;
+    slot[4 * pos] = kind;
+    slot[4 * pos + 1] = i0;
+    slot[4 * pos + 2] = i1;
+    slot[4 * pos + 3] = float_bits(f);
+  };
+  for (int32_t i = 0; i < half; ++i) {
+    claim(ds->diag_first_pos[i], kSlotConst, float_bits(1.0 + ds->eps), 0, 1.0);
+    claim(ds->diag_second_pos[i], kSlotConst, float_bits(-ds->eps), 0, 0.0);
+  }
+  for (int32_t k = 0; k < ds->n_p_full; ++k) {
+    const int32_t v = ds->p_full_value_map[k];
+    WMPC_REQUIRE(v >= 0 && v < ds->nnz_p, "p_full_value_map out of range");
+    claim(ds->p_block_pos[k], kSlotP, v, 0, 1.0);
+  }
+  for (int32_t k = 0; k < ds->n_g; ++k) {
+    const int32_t v = ds->g_a_value_map[k], r = ds->g_rows[k], c = ds->g_cols[k];
+    WMPC_REQUIRE(v >= 0 && v < ds->nnz_a && r >= 0 && r < ni && c >= 0 && c < n, "G entry out of range");
+    claim(ds->g_block_pos[k], kSlotA, v, 0, ds->g_sign[k]);
+    claim(ds->glambda_block_pos[k], kSlotGLambda, v, r, ds->g_sign[k]);
+  }
+  for (int32_t k = 0; k < ds->n_aeq; ++k) {
+    const int32_t v = ds->aeq_a_value_map[k];
+    WMPC_REQUIRE(v >= 0 && v < ds->nnz_a, "aeq_a_value_map out of range");
+    claim(ds->aeq_block_pos[k], kSlotA, v, 0, 1.0);
+    claim(ds->aeq_t_block_pos[k], kSlotA, v, 0, 1.0);
+  }
+  for (int32_t i = 0; i < ni; ++i) claim(ds->slack_diag_pos[i], kSlotSlack, i, 0, 0.0);
+  WMPC_REQUIRE(err.empty(), err.c_str());
+  // rows of the full symmetric M0: the row's upper entries (columns >= o,
+  // ascending), then the mirrored ones (rows < o, ascending) -- the order in
+  // which the reference's scatter matvec accumulates them
+  std::vector<int32_t> mv_ptr(dim + 1, 0), mv;
+  {
+    std::vector<std::vector<int32_t>> up(dim), lo(dim);
+    for (int32_t k = 0; k < nz; ++k) {
+      up[rows[k]].push_back(k);
+      if (rows[k] != cols[k]) lo[cols[k]].push_back(k);
+    }
+    for (int32_t o = 0; o < dim; ++o) {
+      for (int32_t k : up[o]) mv.insert(mv.end(), {cols[k], k});
+      for (int32_t k : lo[o]) mv.insert(mv.end(), {rows[k], k});
+      mv_ptr[o + 1] = (int32_t)(mv.size() / 2);
+    }
+  }
+  // rows of G (entries within a row by ascending column, as the scatter of
+  // solver.py:1089-1092 accumulates them)
+  std::vector<int32_t> g_ptr(ni + 1, 0), g;
+  {
+    std::vector<std::vector<int32_t>> byrow(ni);
+    for (int32_t k = 0; k < ds->n_g; ++k) byrow[ds->g_rows[k]].push_back(k);
+    for (int32_t i = 0; i < ni; ++i) {
+      std::stable_sort(byrow[i].begin(), byrow[i].end(),
+                       [&](int32_t a, int32_t b) { return ds->g_cols[a] < ds->g_cols[b]; });
+      for (int32_t k : byrow[i])
+        g.insert(g.end(), {ds->g_cols[k], ds->g_a_value_map[k], float_bits(ds->g_sign[k]), 0});
+      g_ptr[i + 1] = (int32_t)(g.size() / 4);
+    }
+  }
+  std::vector<int32_t> ineq(2 * (size_t)ni), rsrc(2 * (size_t)dim, 0), con(4 * (size_t)m, -1);
+  for (int32_t i = 0; i < m; ++i) con[4 * i + 3] = 0;
+  for (int32_t o = 0; o < dim; ++o) rsrc[2 * o] = kRhsZero;
+  for (int32_t i = 0; i < n; ++i) {
+    rsrc[2 * i] = kRhsDx;
+    rsrc[2 * i + 1] = i;
+  }
+  for (int32_t i = 0; i < ni; ++i) {
+    const bool upper = i >= nl;
+    const int32_t c = upper ? ds->upper_indices[i - nl] : ds->lower_indices[i];
+    WMPC_REQUIRE(c >= 0 && c < m, "lower/upper index out of range");
+    ineq[2 * i] = c;
+    ineq[2 * i + 1] = upper ? 1 : 0;
+    rsrc[2 * (n + i)] = kRhsDy;
+    rsrc[2 * (n + i) + 1] = c;
+    con[4 * c + (upper ? 1 : 0)] = half + n + i;
+  }
+  for (int32_t i = 0; i < ne; ++i) {
+    const int32_t c = ds->eq_indices[i];
+    WMPC_REQUIRE(c >= 0 && c < m, "eq index out of range");
+    WMPC_REQUIRE(con[4 * c] < 0 && con[4 * c + 1] < 0, "an equality row is also an inequality row");
+    rsrc[2 * (n + ni + i)] = kRhsEq;
+    rsrc[2 * (n + ni + i) + 1] = c;
+    con[4 * c + 2] = half + n + ni + i;
+  }
+  std::vector<int32_t> pent(2 * (size_t)ds->nnz_p), aent(2 * (size_t)ds->nnz_a);
+  for (int32_t k = 0; k < ds->nnz_p; ++k) {
+    WMPC_REQUIRE(ds->p_upper_rows[k] >= 0 && ds->p_upper_rows[k] < n && ds->p_upper_cols[k] >= 0 &&
+                     ds->p_upper_cols[k] < n,
+                 "p_upper_rows/cols out of range");
+    pent[2 * k] = ds->p_upper_rows[k];
+    pent[2 * k + 1] = ds->p_upper_cols[k];
+  }
+  for (int32_t k = 0; k < ds->nnz_a; ++k) {
+    WMPC_REQUIRE(ds->a_rows[k] >= 0 && ds->a_rows[k] < m && ds->a_cols[k] >= 0 && ds->a_cols[k] < n,
+                 "a_rows/a_cols out of range");
+    aent[2 * k] = ds->a_rows[k];
+    aent[2 * k + 1] = ds->a_cols[k];
+  }
+  Blob blob;
+  const size_t t_slot = blob.add(slot), t_mvp = blob.add(mv_ptr), t_mv = blob.add(mv),
+               t_gp = blob.add(g_ptr), t_g = blob.add(g), t_in = blob.add(ineq), t_rs = blob.add(rsrc),
+               t_con = blob.add(con), t_pe = blob.add(pent), t_ae = blob.add(aent);
+  auto* plan = new wmpc_adjoint_plan();
+  if (int rc = blob.upload(&plan->block, "adjoint plan")) {
+    delete plan;
+    return rc;
+  }
+  AdjDev& d = plan->d;
+  d.n = n;
+  d.m = m;
+  d.ni = ni;
+  d.half = half;
+  d.dim = dim;
+  d.nnz_p = ds->nnz_p;
+  d.nnz_a = ds->nnz_a;
+  d.slot = blob.at<int4>(t_slot);
+  d.mv_ptr = blob.at<int32_t>(t_mvp);
+  d.mv = blob.at<int2>(t_mv);
+  d.g_ptr = blob.at<int32_t>(t_gp);
+  d.g = blob.at<int4>(t_g);
+  d.ineq = blob.at<int2>(t_in);
+  d.rsrc = blob.at<int2>(t_rs);
+  d.con = blob.at<int4>(t_con);
+  d.pent = blob.at<int2>(t_pe);
+  d.aent = blob.at<int2>(t_ae);
+  plan->adj = adj;
+  *out = plan;
+  return WMPC_OK;
+}
+
+int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan) {
+  clear_error();
+  if (!plan) return WMPC_OK;
+  if (plan->block) WMPC_HIP_CHECK(hipFree(plan->block));
+  delete plan;
+  return WMPC_OK;
+}
+
+int32_t wmpc_adjoint_dim(const wmpc_adjoint_plan* plan) { return plan ? plan->d.dim : -1; }
+
+int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld) {
+  if (!plan || ld < 0) return -1;
+  return ((int64_t)plan->adj->host.nnz_l + 4 * (int64_t)plan->d.dim + 2 * (int64_t)plan->d.ni) * ld;
+}
+
+int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
+                          const float* a_values, int64_t pm_ld, const float* l, const float* u,
+                          const float* x, const float* y, const float* dx, const float* dy,
+                          int32_t refine, float* work, int64_t work_len, float* sol, float* dq,
+                          float* dl, float* du, float* dp, float* da, int64_t batch, int64_t ld,
+                          void* stream) {
+  clear_error();
+  WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && l && u && x && y && dx && dy && work &&
+                                       dq && dl && du)),
+               "null argument");
+  WMPC_REQUIRE(batch >= 0 && ld >= batch, "need 0 <= batch <= ld");
+  WMPC_REQUIRE(pm_ld == 0 || pm_ld >= batch, "pm_ld must be 0 (shared) or >= batch");
+  WMPC_REQUIRE(refine >= 0, "refine must be nonnegative");
+  if (batch == 0) return WMPC_OK;
+  WMPC_REQUIRE(work_len >= wmpc_adjoint_work_floats(plan, ld),
+               "work is smaller than wmpc_adjoint_work_floats(plan, ld)");
+  const QdldlProg& P = plan->adj->prog;
+  const AdjDev& dv = plan->d;
+  const int64_t dim = dv.dim, nl = P.nnz_l;
+  float* lx = work;
+  float* dinv = lx + nl * ld;
+  float* stash = dinv + dim * ld;
+  float* rbuf = stash + dim * ld;
+  float* solw = rbuf + dim * ld;
+  float* lam = solw + dim * ld;
+  float* slk = lam + (int64_t)dv.ni * ld;
+  const size_t lds = (size_t)adjoint_lane_slots(P) * kWave * sizeof(float) + sweep_wave_bytes();
+  if (lds > kMaxLdsBytes) {
+    set_error("adjoint pattern needs more LDS than a workgroup can have");
+    return WMPC_ERR_UNSUPPORTED;
+  }
+  if (lds > 64 * 1024)
+    WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int64_t grid = (batch + kWave - 1) / kWave;
+  hipLaunchKernelGGL(k_osqp_adjoint, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream, P,
+                     dv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx, dinv, stash, rbuf,
+                     sol ? sol : solw, lam, slk, dq, dl, du, dp, da, refine, batch, ld);
+  return check_launch("osqp adjoint");
+}
+
+}  // extern "C"

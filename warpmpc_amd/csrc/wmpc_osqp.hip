@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "wmpc_blob.h"
 #include "wmpc_plan.h"
 #include "wmpc_stream.h"
 
@@ -309,78 +310,31 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   group_by(m, a_rows, a_cols, a_slot, arow_ptr, arow);
   group_by(n, a_cols, a_rows, a_slot, acol_ptr, acol);
   group_by(n, pf_rows, pf_cols, pf_map, prow_ptr, prow);
-  auto bits = [](double v) {
-    float f = (float)v;
-    int32_t i;
-    std::memcpy(&i, &f, 4);
-    return i;
-  };
   std::vector<int32_t> vtab, ctab;
   for (int32_t c = 0; c < n + m; ++c) {
     const int32_t o = kkt->host.p[c];
     const int32_t ft = kkt->host.ftpos[c];
     if (o < n) {
-      vtab.insert(vtab.end(), {c, o, ft, bits(ds->q_scale[o])});
+      vtab.insert(vtab.end(), {c, o, ft, float_bits(ds->q_scale[o])});
     } else {
       const int32_t i = o - n;
-      ctab.insert(ctab.end(), {c, i, ft, bits(ds->rho_inv_vec[i]), bits(ds->rho_vec[i]),
-                               bits(ds->scaling_e[i]), 0, 0});
+      ctab.insert(ctab.end(), {c, i, ft, float_bits(ds->rho_inv_vec[i]), float_bits(ds->rho_vec[i]),
+                               float_bits(ds->scaling_e[i]), 0, 0});
     }
   }
-  // one host blob: every sub-array 256-byte aligned
-  std::vector<uint32_t> blob;
-  std::vector<size_t> offs;
-  auto app_i = [&](const int32_t* s, size_t c) {
-    offs.push_back(blob.size());
-    for (size_t i = 0; i < c; ++i) blob.push_back((uint32_t)s[i]);
-    blob.push_back(0);
-    while (blob.size() % 64) blob.push_back(0);
-  };
-  auto app_f = [&](const float* s, size_t c) {
-    offs.push_back(blob.size());
-    for (size_t i = 0; i < c; ++i) {
-      uint32_t v;
-      std::memcpy(&v, &s[i], 4);
-      blob.push_back(v);
-    }
-    blob.push_back(0);
-    while (blob.size() % 64) blob.push_back(0);
-  };
-  auto app_d = [&](const double* s, size_t c) {
-    std::vector<float> v(c);
-    for (size_t i = 0; i < c; ++i) v[i] = (float)s[i];
-    app_f(v.data(), c);
-  };
-  app_i(kind.data(), kind.size());
-  app_i(idx.data(), idx.size());
-  app_f(scale.data(), scale.size());
-  app_f(off.data(), off.size());
-  app_i(vtab.data(), vtab.size());
-  app_i(ctab.data(), ctab.size());
-  app_d(ds->scaling_d, n);
-  app_d(ds->scaling_dinv, n);
-  app_d(ds->scaling_e, m);
-  app_d(ds->scaling_einv, m);
-  app_d(ds->y_scale, m);
-  app_d(ds->y_unscale, m);
-  app_i(arow_ptr.data(), arow_ptr.size());
-  app_i(arow.data(), arow.size());
-  app_i(acol_ptr.data(), acol_ptr.size());
-  app_i(acol.data(), acol.size());
-  app_i(prow_ptr.data(), prow_ptr.size());
-  app_i(prow.data(), prow.size());
-
+  Blob blob;
+  const size_t t_kind = blob.add(kind), t_idx = blob.add(idx), t_scale = blob.add(scale),
+               t_off = blob.add(off), t_vtab = blob.add(vtab), t_ctab = blob.add(ctab);
+  const size_t t_d = blob.add_f32(ds->scaling_d, n), t_dinv = blob.add_f32(ds->scaling_dinv, n),
+               t_e = blob.add_f32(ds->scaling_e, m), t_einv = blob.add_f32(ds->scaling_einv, m),
+               t_ys = blob.add_f32(ds->y_scale, m), t_yu = blob.add_f32(ds->y_unscale, m);
+  const size_t t_arp = blob.add(arow_ptr), t_ar = blob.add(arow), t_acp = blob.add(acol_ptr),
+               t_ac = blob.add(acol), t_prp = blob.add(prow_ptr), t_pr = blob.add(prow);
   auto* plan = new wmpc_osqp_plan();
-  hipError_t e = hipMalloc(&plan->block, blob.size() * 4);
-  if (e == hipSuccess) e = hipMemcpy(plan->block, blob.data(), blob.size() * 4, hipMemcpyHostToDevice);
-  if (e != hipSuccess) {
-    set_error(std::string("osqp plan upload: ") + hipGetErrorString(e));
-    if (plan->block) (void)hipFree(plan->block);
+  if (int rc = blob.upload(&plan->block, "osqp plan")) {
     delete plan;
-    return WMPC_ERR_HIP;
+    return rc;
   }
-  const uint32_t* base = static_cast<const uint32_t*>(plan->block);
-  size_t i = 0;
   OsqpDev& d = plan->d;
   d.n = n;
   d.m = m;
@@ -389,26 +343,26 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   d.nnz_kkt = ds->nnz_kkt;
   d.sigma = (float)ds->sigma;
   d.alpha = (float)ds->alpha;
-  d.src_kind = (const int32_t*)(base + offs[i++]);
-  d.src_idx = (const int32_t*)(base + offs[i++]);
-  d.src_scale = (const float*)(base + offs[i++]);
-  d.src_off = (const float*)(base + offs[i++]);
+  d.src_kind = blob.at<int32_t>(t_kind);
+  d.src_idx = blob.at<int32_t>(t_idx);
+  d.src_scale = blob.at<float>(t_scale);
+  d.src_off = blob.at<float>(t_off);
   d.nvar = n;
   d.ncon = m;
-  d.vtab = (const int4*)(base + offs[i++]);
-  d.ctab = (const int4*)(base + offs[i++]);
-  d.sd = (const float*)(base + offs[i++]);
-  d.sdinv = (const float*)(base + offs[i++]);
-  d.se = (const float*)(base + offs[i++]);
-  d.seinv = (const float*)(base + offs[i++]);
-  d.y_scale = (const float*)(base + offs[i++]);
-  d.y_unscale = (const float*)(base + offs[i++]);
-  d.arow_ptr = (const int32_t*)(base + offs[i++]);
-  d.arow = (const int2*)(base + offs[i++]);
-  d.acol_ptr = (const int32_t*)(base + offs[i++]);
-  d.acol = (const int2*)(base + offs[i++]);
-  d.prow_ptr = (const int32_t*)(base + offs[i++]);
-  d.prow = (const int2*)(base + offs[i++]);
+  d.vtab = blob.at<int4>(t_vtab);
+  d.ctab = blob.at<int4>(t_ctab);
+  d.sd = blob.at<float>(t_d);
+  d.sdinv = blob.at<float>(t_dinv);
+  d.se = blob.at<float>(t_e);
+  d.seinv = blob.at<float>(t_einv);
+  d.y_scale = blob.at<float>(t_ys);
+  d.y_unscale = blob.at<float>(t_yu);
+  d.arow_ptr = blob.at<int32_t>(t_arp);
+  d.arow = blob.at<int2>(t_ar);
+  d.acol_ptr = blob.at<int32_t>(t_acp);
+  d.acol = blob.at<int2>(t_ac);
+  d.prow_ptr = blob.at<int32_t>(t_prp);
+  d.prow = blob.at<int2>(t_pr);
   plan->kkt = kkt;
   *out = plan;
   return WMPC_OK;

@@ -19,14 +19,11 @@ diagonals (the matrix the iterative refinement converges to).  The block
 layout, the entry order inside each block and ``eps = 1e-6`` follow the
 reference so that ``adjoint_upper``'s structure matches it.
 
-Besides the reference's ``OSQPDerivativePlan`` fields this module derives the
-per-slot value sources and gather tables the HIP kernel and the CPU oracle
-evaluate (:class:`AdjointTables`).
+The HIP library derives its per-slot value sources and gather tables from
+these fields (``wmpc_adjoint_plan_create``).
 """
 
 from __future__ import annotations
-
-from dataclasses import dataclass
 
 import numpy as np
 import scipy.sparse as sp
@@ -37,13 +34,6 @@ from .types import OSQPDerivativePlan
 OSQP_INFTY = 1e30
 OSQP_MIN_SCALING = 1e-4
 ADJOINT_EPS = 1e-6
-
-# value sources of an adjoint slot (kind, i0, i1, f)
-SLOT_CONST = 0     # f
-SLOT_P = 1         # p_values[i0]
-SLOT_A = 2         # a_values[i0] * f
-SLOT_GLAMBDA = 3   # lambda[i1] * a_values[i0] * f   (i1: inequality row)
-SLOT_SLACK = 4     # (G x)[i0] - h[i0]
 
 
 def _csc_rows_cols(matrix: sp.csc_matrix):
@@ -77,27 +67,6 @@ def selected_rows_pattern(a_csc: sp.csc_matrix, selected: np.ndarray, row_offset
     vidx = np.flatnonzero(keep)
     order = np.lexsort((out_r, out_c))
     return out_r[order].astype(np.int32), out_c[order].astype(np.int32), vidx[order].astype(np.int32)
-
-
-@dataclass(frozen=True)
-class AdjointTables:
-    """Per-slot value sources and gather tables of the adjoint kernel / oracle."""
-
-    slot_kind: np.ndarray   # [nnz_adj] SLOT_*
-    slot_i0: np.ndarray     # [nnz_adj]
-    slot_i1: np.ndarray     # [nnz_adj]
-    slot_f: np.ndarray      # [nnz_adj] float64
-    slot_pert: np.ndarray   # [nnz_adj] +eps / -eps on the two diagonals, else 0
-    mv_ptr: np.ndarray      # [dim + 1] rows of the full symmetric M0
-    mv_col: np.ndarray      # column of each entry
-    mv_slot: np.ndarray     # adjoint slot holding its value
-    g_ptr: np.ndarray       # [n_ineq + 1] rows of G
-    g_col: np.ndarray       # column (x index)
-    g_aidx: np.ndarray      # A value index
-    g_sign: np.ndarray      # float64 sign
-    ineq_y: np.ndarray      # [n_ineq] constraint index of each inequality row
-    ineq_upper: np.ndarray  # [n_ineq] 1 for upper-bounded rows
-    eq_idx: np.ndarray      # [n_eq]
 
 
 def build_derivative_plan(p_upper: sp.csc_matrix, a_csc: sp.csc_matrix, l, u, *, ordering: str = "amd",
@@ -210,84 +179,8 @@ def build_derivative_plan(p_upper: sp.csc_matrix, a_csc: sp.csc_matrix, l, u, *,
     )
 
 
-def adjoint_tables(dp: OSQPDerivativePlan) -> AdjointTables:
-    """Value sources of every adjoint slot and the kernel's gather tables."""
-    nnz = dp.adjoint_upper.nnz
-    kind = np.full(nnz, SLOT_CONST, dtype=np.int32)
-    i0 = np.zeros(nnz, dtype=np.int32)
-    i1 = np.zeros(nnz, dtype=np.int32)
-    f = np.zeros(nnz)
-    pert = np.zeros(nnz)
-    f[dp.diag_first_pos] = 1.0
-    pert[dp.diag_first_pos] = dp.eps
-    f[dp.diag_second_pos] = 0.0
-    pert[dp.diag_second_pos] = -dp.eps
-    kind[dp.p_block_pos] = SLOT_P
-    i0[dp.p_block_pos] = dp.p_full_value_map
-    kind[dp.g_block_pos] = SLOT_A
-    i0[dp.g_block_pos] = dp.g_a_value_map
-    f[dp.g_block_pos] = dp.g_sign
-    kind[dp.aeq_block_pos] = SLOT_A
-    i0[dp.aeq_block_pos] = dp.aeq_a_value_map
-    f[dp.aeq_block_pos] = 1.0
-    kind[dp.aeq_t_block_pos] = SLOT_A
-    i0[dp.aeq_t_block_pos] = dp.aeq_a_value_map
-    f[dp.aeq_t_block_pos] = 1.0
-    kind[dp.glambda_block_pos] = SLOT_GLAMBDA
-    i0[dp.glambda_block_pos] = dp.g_a_value_map
-    i1[dp.glambda_block_pos] = dp.g_rows
-    f[dp.glambda_block_pos] = dp.g_sign
-    kind[dp.slack_diag_pos] = SLOT_SLACK
-    i0[dp.slack_diag_pos] = np.arange(dp.n_ineq, dtype=np.int32)
-
-    # rows of the full symmetric M0 (upper entries plus mirrors), gather form
-    m = dp.adjoint_upper
-    dim = dp.adjoint_dim
-    rows = np.asarray(m.indices, dtype=np.int64)
-    cols = np.repeat(np.arange(dim, dtype=np.int64), np.diff(m.indptr))
-    slots = np.arange(rows.size, dtype=np.int64)
-    off = rows != cols
-    fr = np.concatenate([rows, cols[off]])
-    fc = np.concatenate([cols, rows[off]])
-    fs = np.concatenate([slots, slots[off]])
-    order = np.lexsort((fc, fr))
-    mv_ptr = np.zeros(dim + 1, dtype=np.int32)
-    np.cumsum(np.bincount(fr, minlength=dim), out=mv_ptr[1:])
-
-    # rows of G for G x
-    g_order = np.lexsort((dp.g_cols, dp.g_rows))
-    g_ptr = np.zeros(dp.n_ineq + 1, dtype=np.int32)
-    np.cumsum(np.bincount(dp.g_rows, minlength=dp.n_ineq), out=g_ptr[1:])
-    ineq_y = np.concatenate([dp.lower_indices, dp.upper_indices]).astype(np.int32)
-    ineq_upper = np.concatenate([np.zeros(dp.n_ineq_l, np.int32), np.ones(dp.n_ineq_u, np.int32)])
-    return AdjointTables(
-        slot_kind=kind,
-        slot_i0=i0,
-        slot_i1=i1,
-        slot_f=f,
-        slot_pert=pert,
-        mv_ptr=mv_ptr,
-        mv_col=fc[order].astype(np.int32),
-        mv_slot=fs[order].astype(np.int32),
-        g_ptr=g_ptr,
-        g_col=np.asarray(dp.g_cols, dtype=np.int32)[g_order],
-        g_aidx=np.asarray(dp.g_a_value_map, dtype=np.int32)[g_order],
-        g_sign=np.asarray(dp.g_sign, dtype=np.float64)[g_order],
-        ineq_y=ineq_y,
-        ineq_upper=ineq_upper,
-        eq_idx=np.asarray(dp.eq_indices, dtype=np.int32),
-    )
-
-
 __all__ = [
     "ADJOINT_EPS",
-    "AdjointTables",
-    "SLOT_A",
-    "SLOT_CONST",
-    "SLOT_GLAMBDA",
-    "SLOT_P",
-    "SLOT_SLACK",
-    "adjoint_tables",
     "build_derivative_plan",
     "full_symmetric_pattern",
     "selected_rows_pattern",
