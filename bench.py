@@ -1,28 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched fixed-pattern QDLDL factor + solve on one MI355X.
+"""Benchmark of the batched fixed-pattern QP core on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): the KKT matrix
-[P + sigma I, Aᵀ; A, -diag(1/rho)] of the mass-chain linear MPC (6 unit masses,
-nx = 12, nu = 4, springs k = 1, ZOH dt = 0.5) at horizon N = 50, batch 65,536,
-float32, per-instance values (P diagonal 1 + U[0,1], rho log-uniform in
-[1e-3, 1e2] per constraint), rhs ~ N(0, 1), seed 1.  One step = one fused
-factor + solve launch over the whole batch, inputs resident in HBM.
+Headline (``value``): plain forward linear-MPC QP solves/s.  Workload: the
+mass-chain MPC (6 unit masses, nx = 12, nu = 4, springs k = 1, ZOH dt = 0.5,
+horizon N = 10, Q = I, R = I, |x| <= 4, |u| <= 0.5; n = 172 variables, m = 304
+constraints), P and A shared by the batch, x0 ~ U[-1, 1]^12 per instance
+(seed 0), batch 65,536, float32, OSQP rho = 0.1 (vector rho), sigma = 1e-6,
+alpha = 1.6, 25 ADMM iterations, cold start.  One step = one
+``wmpc_osqp_solve_f32`` launch set (the shared KKT factor, then the fused
+ADMM kernel) over the whole batch, with inputs resident in HBM in the
+library's batch-major layout.
 
-Also reported (extra keys of the same JSON line):
-* ``osqp_fwd``: plain forward linear-MPC QP solves/s (mass chain N = 10,
-  batch 65,536, 25 ADMM iterations, shared P/A);
-* ``err_vs_fp64``: worst-case and 99.9th-percentile per-instance relative error
-  of x against the FP64 oracle on a sample of instances;
-* ``roofline``: algorithmic HBM bytes per launch / measured kernel time;
-* ``cpu_baseline``: the oracle's float32 CPU counterpart (OpenMP) on a bounded
-  sample of the same workload.
+Extra legs on the same JSON line:
+* ``fwd_adjoint``: forward + adjoint QP solves/s on the cart-pole MPC
+  (N = 30, |u| <= 10, Q = diag(exp theta_q), R = exp theta_r per instance,
+  batch 16,384): the forward solve, the loss cotangent, and one fused adjoint
+  launch with the reference's 100 refinement steps;
+* ``qdldl``: the QDLDL factor + solve microbenchmark (mass chain N = 50,
+  per-instance KKT values, batch 65,536);
+* ``osqp_per_instance``: the headline QP with per-instance P and A, so the
+  factor is computed and streamed per instance;
+* ``err_vs_fp64`` of each leg against the FP64 oracle on sampled instances;
+* ``cpu_baseline``: the oracle's float32 OSQP (OpenMP over instances) on a
+  bounded sample of the headline workload, all host cores and one core.
 """
 
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import pathlib
 import re
 import sys
@@ -34,6 +40,7 @@ if str(ROOT) not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "QP solves/s (fwd; fwd+adjoint) at batch 64k, err≤1e-4 vs FP64, % HBM peak"
+S = 4  # bytes per float32
 
 
 def _parse():
@@ -42,13 +49,17 @@ def _parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--horizon", type=int, default=50)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-osqp", action="store_true", help="skip the OSQP forward leg")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--legs", default="osqp,adjoint,qdldl,per_instance,cpu",
+                    help="comma-separated subset of osqp,adjoint,qdldl,per_instance,cpu")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--refine", type=int, default=100, help="adjoint refinement steps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return ap.parse_args()
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def _copy_calibration(steps: int) -> int:
@@ -65,13 +76,346 @@ def _copy_calibration(steps: int) -> int:
     return 0
 
 
-def _traffic_from_profile(name: str):
-    """HBM bytes per launch recorded by the committed rocprofv3 PMC summary, or None."""
+def _profile_value(name: str, key: str):
+    """A number recorded in a committed rocprofv3 summary under profiles/, or None."""
     path = ROOT / "profiles" / name
     if not path.exists():
         return None
-    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", path.read_text(), re.M)
+    m = re.search(rf"^{key}:\s*([0-9.eE+-]+)", path.read_text(), re.M)
     return float(m.group(1)) if m else None
+
+
+def _timed(step, steps: int, warmup: int, stream):
+    """Warm up, then time `steps` calls with a synchronize on both sides (wall)
+    and HIP events on the launch stream around each call (device time)."""
+    import numpy as np
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dev_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
+    return wall, dev_ms
+
+
+def _rel_err(v, ref):
+    """Per instance max |v - v64| / max(||v64||_inf, 1e-30)."""
+    import numpy as np
+
+    v = np.asarray(v, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    return np.max(np.abs(v - ref), axis=1) / np.maximum(np.max(np.abs(ref), axis=1), 1e-30)
+
+
+def _err_summary(errs: dict) -> dict:
+    import numpy as np
+
+    out = {}
+    for k, e in errs.items():
+        out[k] = {"max": float(np.max(e)), "p999": float(np.percentile(e, 99.9)),
+                  "above_1e-4": int(np.sum(e > 1e-4))}
+    return out
+
+
+def _sym(t):
+    """(B, w) batch-first -> contiguous (w, B) batch-major."""
+    return t.t().contiguous()
+
+
+# --------------------------------------------------------------------------- headline: plain forward QP
+
+
+def _chain_problem():
+    import numpy as np
+
+    from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan
+    from warpmpc_amd.problems import mass_chain_mpc
+
+    prob = mass_chain_mpc(horizon=10)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
+    return np, prob, plan
+
+
+def _osqp_setup(plan, per_instance: bool):
+    from warpmpc_amd.osqp import compile_osqp
+
+    solver = compile_osqp(plan, "float32", qdldl_backend="hip", transpose_work=True)
+    return solver, solver.extras["device_plan"]
+
+
+def _u2_bytes(plan, k: int) -> int:
+    """U2 model: read P, A, q, l, u; write L, Dinv; K iterations of both sweeps
+    over L plus Dinv; write x, z, y, prim, dual, obj."""
+    n, m = plan.n, plan.m
+    nl, nk = plan.qdldl_plan.nnz_l, plan.qdldl_plan.n
+    return S * ((plan.nnz_p + plan.nnz_a) + (n + 2 * m) + (nl + nk) + k * (2 * nl + nk) + (n + 2 * m + 3))
+
+
+def _osqp_leg(args, per_instance: bool, stream):
+    import torch
+
+    from oracle import oracle
+    from warpmpc_amd import _lib
+    from warpmpc_amd.problems import batched_mpc_data
+
+    np, prob, plan = _chain_problem()
+    b = args.batch
+    _log(f"osqp forward: mass chain N=10 (n={plan.n}, m={plan.m}), batch {b}, 25 iterations, "
+         f"{'per-instance' if per_instance else 'shared'} P/A")
+    solver, dev = _osqp_setup(plan, per_instance)
+    lib = _lib.load()
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=0, dtype=np.float64)
+    if per_instance:
+        rng = np.random.default_rng(10)
+        pv = np.broadcast_to(pv, (b, pv.shape[1])) * rng.uniform(0.5, 2.0, size=(b, pv.shape[1]))
+        av = np.broadcast_to(av, (b, av.shape[1])).copy()
+    f32 = dict(dtype=torch.float32, device="cuda")
+    t = {k: _sym(torch.as_tensor(v, **f32)) for k, v in dict(p=pv, a=av, q=q, l=l, u=u).items()}
+    pm_ld = b if per_instance else 0
+    n, m = plan.n, plan.m
+    x = torch.empty((n, b), **f32)
+    z = torch.empty((m, b), **f32)
+    y = torch.empty((m, b), **f32)
+    prim, dual, obj = (torch.empty((b,), **f32) for _ in range(3))
+    mode = 2 if per_instance else 1
+    work = torch.empty((int(lib.wmpc_osqp_work_floats(dev.handle, mode, b)),), **f32)
+    ptr = lambda v: None if v is None else v.data_ptr()  # noqa: E731
+
+    def step():
+        _lib.check(lib.wmpc_osqp_solve_f32(
+            dev.handle, ptr(t["p"]), ptr(t["a"]), pm_ld, ptr(t["q"]), ptr(t["l"]), ptr(t["u"]), None, None, None,
+            None, None, 0, ptr(work), work.numel(), ptr(x), ptr(z), ptr(y), ptr(prim), ptr(dual), ptr(obj),
+            25, b, b, stream.cuda_stream), "wmpc_osqp_solve_f32")
+
+    wall, dev_ms = _timed(step, args.steps, args.warmup, stream)
+    # accuracy of every output against the FP64 oracle on sampled instances
+    cols = np.random.default_rng(0).choice(b, size=min(256, b), replace=False)
+    orc = oracle.OracleOSQP(plan)
+    p_in = pv[cols] if per_instance else pv
+    a_in = av[cols] if per_instance else av
+    p32 = np.asarray(p_in, dtype=np.float32).astype(np.float64)
+    a32 = np.asarray(a_in, dtype=np.float32).astype(np.float64)
+    ref = orc.solve(p32, a32, q[cols].astype(np.float32), l[cols].astype(np.float32), u[cols].astype(np.float32))
+    got = [v[:, cols].t().cpu().numpy() for v in (x, z, y)]
+    errs = {k: _rel_err(g, r) for k, g, r in zip(("x", "z", "y"), got, ref[:3])}
+    return dict(plan=plan, wall=wall, dev_ms=dev_ms, value=b * args.steps / wall, errs=_err_summary(errs))
+
+
+# --------------------------------------------------------------------------- fwd + adjoint (cart-pole)
+
+
+def _adjoint_leg(args, stream):
+    import numpy as np
+    import torch
+
+    from oracle import oracle
+    from warpmpc_amd import _lib
+    from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
+    from warpmpc_amd.problems import OSQP_INFTY, cartpole_dynamics, mpc_qp
+
+    b, n_h, refine = 16384, 30, args.refine
+    ad, bd = cartpole_dynamics()
+    prob = mpc_qp(ad, bd, n_h, np.ones(4), np.ones(1), x_bound=OSQP_INFTY, u_bound=10.0)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s, derivatives=True)
+    solver = compile_osqp(plan, "float32", derivatives=True, derivative_refinement_iters=refine)
+    dev, adev = solver.extras["device_plan"], solver.extras["adjoint_plan"]
+    lib = _lib.load()
+    dp = plan.derivative_plan
+    _log(f"fwd+adjoint: cart-pole N={n_h} (n={plan.n}, m={plan.m}, adjoint dim {dp.adjoint_dim}), batch {b}, "
+         f"R={refine}")
+    rng = np.random.default_rng(3)
+    tq = rng.normal(0.0, 0.5, size=(b, 4))
+    tr = rng.normal(0.0, 0.5, size=(b, 1))
+    x0 = rng.uniform(-0.2, 0.2, size=(b, 4))
+    pv = np.concatenate([np.tile(np.exp(tq), (1, n_h + 1)), np.tile(np.exp(tr), (1, n_h))], 1)
+    av = np.broadcast_to(plan.a_matrix.data, (b, plan.nnz_a))
+    l = np.tile(prob.l, (b, 1))
+    u = np.tile(prob.u, (b, 1))
+    l[:, :4] = -x0
+    u[:, :4] = -x0
+    n, m = plan.n, plan.m
+    ui = (n_h + 1) * 4
+    f32 = dict(dtype=torch.float32, device="cuda")
+    t = {k: _sym(torch.as_tensor(v, **f32)) for k, v in dict(p=pv, a=av, l=l, u=u).items()}
+    qz = torch.zeros((n, b), **f32)
+    rr = _sym(torch.as_tensor(np.exp(tr), **f32))[0]
+    x = torch.empty((n, b), **f32)
+    z = torch.empty((m, b), **f32)
+    y = torch.empty((m, b), **f32)
+    prim, dual, obj = (torch.empty((b,), **f32) for _ in range(3))
+    xbar = torch.zeros((n, b), **f32)
+    ybar = torch.zeros((m, b), **f32)
+    dq = torch.empty((n, b), **f32)
+    dl = torch.empty((m, b), **f32)
+    du = torch.empty((m, b), **f32)
+    dpm = torch.empty((plan.nnz_p, b), **f32)
+    dam = torch.empty((plan.nnz_a, b), **f32)
+    fwork = torch.empty((int(lib.wmpc_osqp_work_floats(dev.handle, 2, b)),), **f32)
+    awork = torch.empty((int(lib.wmpc_adjoint_work_floats(adev.handle, b)),), **f32)
+
+    def forward():
+        _lib.check(lib.wmpc_osqp_solve_f32(
+            dev.handle, t["p"].data_ptr(), t["a"].data_ptr(), b, qz.data_ptr(), t["l"].data_ptr(),
+            t["u"].data_ptr(), None, None, None, None, None, 0, fwork.data_ptr(), fwork.numel(), x.data_ptr(),
+            z.data_ptr(), y.data_ptr(), prim.data_ptr(), dual.data_ptr(), obj.data_ptr(), 25, b, b,
+            stream.cuda_stream), "wmpc_osqp_solve_f32")
+
+    def adjoint():
+        # loss = sum_b R u0^2 (+ terms without x): its cotangent on x is 2 R u0 at u0
+        xbar[ui].copy_(2.0 * rr * x[ui])
+        _lib.check(lib.wmpc_osqp_adjoint_f32(
+            adev.handle, t["p"].data_ptr(), t["a"].data_ptr(), b, t["l"].data_ptr(), t["u"].data_ptr(),
+            x.data_ptr(), y.data_ptr(), xbar.data_ptr(), ybar.data_ptr(), refine, awork.data_ptr(), awork.numel(),
+            None, dq.data_ptr(), dl.data_ptr(), du.data_ptr(), dpm.data_ptr(), dam.data_ptr(), b, b,
+            stream.cuda_stream), "wmpc_osqp_adjoint_f32")
+
+    def step():
+        forward()
+        adjoint()
+
+    wall, dev_ms = _timed(step, max(2, args.steps // 2), args.warmup, stream)
+    steps = max(2, args.steps // 2)
+    _, fwd_ms = _timed(forward, steps, 1, stream)
+    _, adj_ms = _timed(adjoint, steps, 1, stream)
+    # the same adjoint in the reference's algorithm by the oracle, FP64 and FP32
+    cols = np.random.default_rng(1).choice(b, size=96, replace=False)
+    f64 = lambda v: v[:, cols].t().cpu().numpy().astype(np.float64)  # noqa: E731
+    args_o = (np.asarray(pv[cols], np.float32).astype(np.float64), plan.a_matrix.data[None],
+              np.asarray(l[cols], np.float32).astype(np.float64), np.asarray(u[cols], np.float32).astype(np.float64),
+              f64(x), f64(y), f64(xbar), f64(ybar))
+    adj = oracle.OracleAdjoint(plan, refine=refine)
+    r64 = adj.compute(*args_o)[3:]
+    r32 = adj.compute(*args_o, dtype=np.float32)[3:]
+    got = [f64(v) for v in (dq, dl, du, dpm, dam)]
+    names = ("dq", "dl", "du", "dp", "da")
+    errs = {k: _rel_err(g, r) for k, g, r in zip(names, got, r64)}
+    spread = {k: _rel_err(a, r) for k, a, r in zip(names, r32, r64)}
+    # U3: B_qp + s [n + 2 (nnz_P + nnz_A) + (n + 2m) + R (2 nnz_L' + n_K') + F (nnz_K' + nnz_L' + n_K')]
+    q2 = dp.qdldl_plan
+    u2 = _u2_bytes(plan, 25)
+    u3 = u2 + S * (n + 2 * (plan.nnz_p + plan.nnz_a) + (n + 2 * m) + (refine + 1) * (2 * q2.nnz_l + q2.n)
+                   + 1 * (q2.nnz_a + q2.nnz_l + q2.n))
+    return dict(b=b, wall=wall, dev_ms=dev_ms, fwd_ms=fwd_ms, adj_ms=adj_ms, steps=steps, u3=u3, u2=u2,
+                plan=plan, dp=dp, errs=_err_summary(errs), spread=_err_summary(spread), refine=refine)
+
+
+# --------------------------------------------------------------------------- QDLDL microbench (C1)
+
+
+def _qdldl_leg(args, stream):
+    import numpy as np
+    import torch
+
+    from oracle import oracle
+    from warpmpc_amd import _lib
+    from warpmpc_amd.problems import kkt_microbench_plan, kkt_microbench_values
+    from warpmpc_amd.qdldl import compile_qdldl_variant
+
+    b = args.batch
+    _log(f"qdldl factor+solve: mass-chain KKT N=50, batch {b}")
+    _, oplan = kkt_microbench_plan(horizon=50)
+    q = oplan.qdldl_plan
+    comp = compile_qdldl_variant(q, np.float32, backend="hip", transpose_work=True)
+    dev = comp.extras["device_plan"]
+    lib = _lib.load()
+    vals, rhs = kkt_microbench_values(oplan, b, seed=1, device="cuda")
+    f32 = dict(dtype=torch.float32, device="cuda")
+    x = torch.empty((q.n, b), **f32)
+    lx = torch.empty((q.nnz_l, b), **f32)
+    d = torch.empty((q.n, b), **f32)
+    dinv = torch.empty((q.n, b), **f32)
+    work = torch.empty((q.n, b), **f32)
+
+    def step():
+        _lib.check(lib.wmpc_qdldl_factor_solve_f32(dev.handle, vals.data_ptr(), rhs.data_ptr(), x.data_ptr(),
+                                                   lx.data_ptr(), d.data_ptr(), dinv.data_ptr(), work.data_ptr(),
+                                                   work.numel(), b, b, stream.cuda_stream), "factor_solve")
+
+    wall, dev_ms = _timed(step, args.steps, args.warmup, stream)
+    cols = np.random.default_rng(0).choice(b, size=min(256, b), replace=False)
+    v64 = vals[:, cols].cpu().numpy().T.astype(np.float64)
+    r64 = rhs[:, cols].cpu().numpy().T.astype(np.float64)
+    lx64, _, dinv64 = oracle.qdldl_factor(q, v64)
+    x64 = oracle.qdldl_solve(q, lx64, dinv64, r64)
+    rel = _rel_err(x[:, cols].cpu().numpy().T, x64)
+    # U1: read values + rhs; write L, D, Dinv; re-read L in the backward sweep; write x
+    u1 = S * (q.nnz_a + 2 * q.nnz_l + 4 * q.n)
+    ach = u1 * b / (dev_ms * 1e-3) / 1e9
+    return {"value": b * args.steps / wall, "unit": "factor+solve systems/s", "batch": b, "kernel_ms": dev_ms,
+            "config": f"mass-chain MPC KKT N=50 (n_kkt={q.n}, nnz_kkt={q.nnz_a}, nnz_L={q.nnz_l}), "
+                      f"per-instance values",
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "bytes_per_system": u1,
+                         "traffic": _profile_value("qdldl_factor_solve_n50_b65536_pmc.txt",
+                                                   "traffic_bytes_per_launch")},
+            "err_vs_fp64": _err_summary({"x": rel})}
+
+
+# --------------------------------------------------------------------------- CPU baseline
+
+
+def _cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_leg(args, plan):
+    import os
+
+    import numpy as np
+
+    from oracle import oracle
+    from warpmpc_amd.problems import batched_mpc_data, mass_chain_mpc
+
+    prob = mass_chain_mpc(horizon=10)
+    chunk = 1024
+    pv, av, q, l, u = batched_mpc_data(prob, chunk, seed=0, dtype=np.float32)
+    orc = oracle.OracleOSQP(plan)
+    cores = len(os.sched_getaffinity(0))
+    out = {}
+    for threads, secs in ((cores, args.cpu_seconds), (1, args.cpu_seconds / 3)):
+        oracle.set_threads(threads)
+        orc.solve(pv, av, q[:64], l[:64], u[:64], dtype=np.float32)  # warm-up
+        _log(f"cpu baseline: float32 oracle OSQP, {threads} thread(s), >= {secs:.0f} s")
+        done, reps = 0, []
+        t_end = time.perf_counter() + secs
+        while True:
+            t0 = time.perf_counter()
+            orc.solve(pv, av, q, l, u, dtype=np.float32)
+            reps.append(time.perf_counter() - t0)
+            done += chunk
+            if time.perf_counter() >= t_end and len(reps) >= 5:
+                break
+        out[threads] = (chunk / float(np.median(reps)), done, sum(reps), len(reps))
+    oracle.set_threads(cores)
+    v_all, done_all, t_all, n_all = out[cores]
+    v_one, done_one, t_one, n_one = out[1]
+    return {"value": v_all, "unit": "QP solves/s", "cores": cores, "kind": "port",
+            "one_core_value": v_one, "cpu_model": _cpu_model(),
+            "sample": f"the headline workload (mass chain N=10, 25 ADMM iterations, shared P/A) in chunks of "
+                      f"{chunk} instances, value = chunk / median chunk time: {n_all} chunks ({done_all} QPs, "
+                      f"{t_all:.1f} s) on {cores} cores, {n_one} chunks ({done_one} QPs, {t_one:.1f} s) on 1 core; "
+                      f"float32 restatement of the reference's op order, OpenMP over instances"}
+
+
+# --------------------------------------------------------------------------- main
 
 
 def main() -> int:
@@ -82,174 +426,77 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         return _copy_calibration(args.steps)
-    import numpy as np
     import torch
 
-    from warpmpc_amd import _lib
-    from warpmpc_amd.problems import kkt_microbench_plan, kkt_microbench_values
-    from warpmpc_amd.qdldl import compile_qdldl_variant
-
+    legs = set(args.legs.split(","))
     torch.cuda.set_device(0)
-    lib = _lib.load()
-    b = args.batch
-    print(f"[bench] planning mass-chain KKT N={args.horizon}", file=sys.stderr, flush=True)
-    _, oplan = kkt_microbench_plan(horizon=args.horizon)
-    q = oplan.qdldl_plan
-    comp = compile_qdldl_variant(q, np.float32, backend="hip", transpose_work=True)
-    dev = comp.extras["device_plan"]
-    vals, rhs = kkt_microbench_values(oplan, b, seed=1, device="cuda")
-    f32 = dict(dtype=torch.float32, device="cuda")
-    x = torch.empty((q.n, b), **f32)
-    lx = torch.empty((q.nnz_l, b), **f32)
-    d = torch.empty((q.n, b), **f32)
-    dinv = torch.empty((q.n, b), **f32)
-    work = torch.empty((q.n, b), **f32)
     stream = torch.cuda.current_stream()
-
-    def step():
-        _lib.check(lib.wmpc_qdldl_factor_solve_f32(dev.handle, vals.data_ptr(), rhs.data_ptr(), x.data_ptr(),
-                                                   lx.data_ptr(), d.data_ptr(), dinv.data_ptr(), work.data_ptr(),
-                                                   work.numel(), b, b, stream.cuda_stream), "factor_solve")
-
-    print(f"[bench] warmup {args.warmup}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        step()
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(starts, ends)]))
-    ms_per_step = wall * 1e3 / args.steps
-    value = b * args.steps / wall
-
-    # accuracy vs the FP64 oracle on a sample of instances
-    from oracle import oracle
-
-    cols = np.random.default_rng(0).choice(b, size=min(256, b), replace=False)
-    v64 = vals[:, cols].cpu().numpy().T.astype(np.float64)
-    r64 = rhs[:, cols].cpu().numpy().T.astype(np.float64)
-    lx64, _, dinv64 = oracle.qdldl_factor(q, v64)
-    x64 = oracle.qdldl_solve(q, lx64, dinv64, r64)
-    xs = x[:, cols].cpu().numpy().T.astype(np.float64)
-    rel = np.max(np.abs(xs - x64), axis=1) / np.max(np.abs(x64), axis=1)
-
-    nnz_k, nnz_l, nk = q.nnz_a, q.nnz_l, q.n
-    # values + rhs in; L, D, Dinv out; L re-read by the backward sweep; x out
-    bytes_per_sys = 4 * (nnz_k + 2 * nnz_l + 4 * nk)
-    achieved = bytes_per_sys * b / (kern_ms * 1e-3) / 1e9
-    traffic = _traffic_from_profile("qdldl_factor_solve_n50_b65536_pmc.txt")
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "factor+solve systems/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"qdldl factor+solve, mass-chain MPC KKT N={args.horizon} (n_kkt={nk}, nnz_kkt={nnz_k}, "
-                        f"nnz_L={nnz_l}), per-instance values, batch {b}",
-            "batch": b,
-            "horizon": args.horizon,
-            "n_kkt": nk,
-            "nnz_kkt": nnz_k,
-            "nnz_l": nnz_l,
-            "parallelism": "single GPU, one lane per instance",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_system": bytes_per_sys,
-            "kernel_ms": kern_ms,
-        },
-        "err_vs_fp64": {"max": float(rel.max()), "p999": float(np.percentile(rel, 99.9)),
-                        "sample": int(cols.size)},
-    }
-
-    if not args.no_osqp:
-        line["osqp_fwd"] = _osqp_leg(args)
-    if not args.no_cpu:
-        line["cpu_baseline"] = _cpu_leg(q, args)
+    b = args.batch
+    line = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic"}
+    plan = None
+    if "osqp" in legs:
+        r = _osqp_leg(args, False, stream)
+        plan = r["plan"]
+        u2 = _u2_bytes(plan, 25)
+        ach = u2 * b / (r["dev_ms"] * 1e-3) / 1e9
+        traffic = _profile_value("osqp_admm_chain_n10_b65536_pmc.txt", "traffic_bytes_per_launch")
+        line.update({
+            "value": r["value"], "unit": "QP solves/s", "ms_per_step": r["wall"] * 1e3 / args.steps,
+            "config": {"workload": f"linear-MPC QP, mass chain nx=12 nu=4 N=10 (n={plan.n}, m={plan.m}, "
+                                   f"nnz_L={plan.qdldl_plan.nnz_l}), shared P/A, x0~U[-1,1]^12 seed 0, "
+                                   f"25 ADMM iterations, cold start",
+                       "batch": b, "max_iter": 25, "parallelism": "single GPU, one lane per instance"},
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "model_bytes_per_qp": u2,
+                         "kernel_ms": r["dev_ms"],
+                         "note": "model U2 counts L and Dinv streamed per instance; with P/A shared by the "
+                                 "batch the factor is one wave-uniform stream read through the scalar cache, "
+                                 "so the model overstates HBM traffic (traffic = measured bytes per launch); "
+                                 "the kernel is instruction-issue/latency bound (one wave per SIMD)"},
+            "err_vs_fp64": r["errs"],
+        })
+    if "adjoint" in legs:
+        r = _adjoint_leg(args, stream)
+        ach = r["u3"] * r["b"] / (r["dev_ms"] * 1e-3) / 1e9
+        q2 = r["dp"].qdldl_plan
+        line["fwd_adjoint"] = {
+            "value": r["b"] * r["steps"] / r["wall"], "unit": "fwd+adjoint QP solves/s", "batch": r["b"],
+            "ms_per_step": r["wall"] * 1e3 / r["steps"], "forward_ms": r["fwd_ms"], "adjoint_ms": r["adj_ms"],
+            "R": r["refine"] + 1, "F": 1,
+            "config": f"cart-pole N=30 (n={r['plan'].n}, m={r['plan'].m}), per-instance Q/R, 25 ADMM iterations; "
+                      f"adjoint of dim {q2.n} (nnz {q2.nnz_a}, nnz_L {q2.nnz_l}) factored per instance, "
+                      f"{r['refine']} refinement steps",
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_qp": r["u3"],
+                         "traffic": _profile_value("osqp_adjoint_cartpole_n30_b16384_pmc.txt",
+                                                   "traffic_bytes_per_launch")},
+            "err_vs_fp64": r["errs"],
+            "reference_fp32_spread": r["spread"],
+        }
+    if "qdldl" in legs:
+        line["qdldl"] = _qdldl_leg(args, stream)
+    if "per_instance" in legs:
+        r = _osqp_leg(args, True, stream)
+        u2 = _u2_bytes(r["plan"], 25)
+        ach = u2 * b / (r["dev_ms"] * 1e-3) / 1e9
+        line["osqp_per_instance"] = {
+            "value": r["value"], "unit": "QP solves/s", "batch": b, "kernel_ms": r["dev_ms"],
+            "config": "headline QP with per-instance P (x U[0.5,2]) and A: factor per instance",
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_qp": u2},
+            "err_vs_fp64": r["errs"],
+        }
+    if "cpu" in legs:
+        if plan is None:
+            _, _, plan = _chain_problem()
+        line["cpu_baseline"] = _cpu_leg(args, plan)
+    if "value" not in line and "fwd_adjoint" in line:
+        line.update(value=line["fwd_adjoint"]["value"], unit=line["fwd_adjoint"]["unit"],
+                    ms_per_step=line["fwd_adjoint"]["ms_per_step"], config={"workload": "fwd+adjoint only"})
     print(json.dumps(line), flush=True)
     return 0
-
-
-def _osqp_leg(args):
-    import numpy as np
-    import torch
-
-    from oracle import oracle
-    from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
-    from warpmpc_amd.problems import batched_mpc_data, mass_chain_mpc
-
-    b = args.batch
-    print("[bench] osqp forward leg: mass chain N=10, 25 iterations", file=sys.stderr, flush=True)
-    prob = mass_chain_mpc(horizon=10)
-    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
-    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
-    solver = compile_osqp(plan, np.float32, qdldl_backend="hip", transpose_work=True)
-    pv, av, q, l, u = batched_mpc_data(prob, b, seed=0, dtype=np.float32)
-    t = [torch.as_tensor(a).cuda() for a in (pv, av, q, l, u)]
-    out = solver.solve(*t)
-    torch.cuda.synchronize()
-    reps = 3
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        out = solver.solve(*t)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
-    cols = np.random.default_rng(0).choice(b, size=128, replace=False)
-    ref = oracle.OracleOSQP(plan).solve(pv, av, q[cols], l[cols], u[cols])
-    xs = out[0].cpu().numpy()[cols].astype(np.float64)
-    rel = np.max(np.abs(xs - ref[0]), axis=1) / np.maximum(np.max(np.abs(ref[0]), axis=1), 1e-30)
-    return {"value": b / dt, "unit": "QP solves/s", "batch": b, "ms": dt * 1e3, "max_iter": 25,
-            "config": "mass chain N=10 (n=172, m=304), shared P/A, rho_is_vec",
-            "err_vs_fp64": {"max": float(rel.max()), "p999": float(np.percentile(rel, 99.9))}}
-
-
-def _cpu_leg(q, args):
-    import numpy as np
-
-    from oracle import oracle
-    from warpmpc_amd.problems import kkt_microbench_plan, kkt_microbench_values
-
-    _, oplan = kkt_microbench_plan(horizon=args.horizon)
-    chunk = 2048
-    vals, rhs = kkt_microbench_values(oplan, chunk, seed=2)
-    v = np.ascontiguousarray(vals.T)
-    r = np.ascontiguousarray(rhs.T)
-    threads = oracle.num_threads()
-    print(f"[bench] cpu baseline: float32 oracle, {threads} threads, >= {args.cpu_seconds:.0f} s",
-          file=sys.stderr, flush=True)
-    done = 0
-    t0 = time.perf_counter()
-    while True:
-        lx, _, dinv = oracle.qdldl_factor(q, v, dtype=np.float32)
-        oracle.qdldl_solve(q, lx, dinv, r, dtype=np.float32)
-        done += chunk
-        if time.perf_counter() - t0 >= args.cpu_seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "factor+solve systems/s", "cores": threads, "kind": "port",
-            "sample": f"{done} systems of the same N={args.horizon} KKT workload in chunks of {chunk}, "
-                      f"float32, OpenMP over instances, {dt:.1f} s"}
 
 
 if __name__ == "__main__":

@@ -89,6 +89,16 @@ int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len
 int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t* out, int64_t cap,
                             int64_t* len);
 
+/* Instances per 64-lane wavefront of every batched kernel: 0 = automatic (64
+ * once the batch gives each SIMD of the chip a wavefront, i.e. batch >= 65536;
+ * below that the smallest of 8, 16, 32 that still does, the lanes of one
+ * instance doing identical work), or 8, 16, 32, 64.  Process-wide.  No
+ * counterpart in the reference (its Warp kernels use one thread per instance):
+ * the occupancy knob of this backend.  wmpc_instances_per_wave reports the
+ * value a launch of `batch` instances uses (-1 on bad input). */
+int wmpc_set_instances_per_wave(int32_t ipw);
+int32_t wmpc_instances_per_wave(int64_t batch);
+
 /* CompiledQDLDL.factor (warp_backend.py:1302-1349; kernel :104-173):
  * a_values [nnz_a][ld] (original upper-CSC order) -> lx [nnz_l][ld],
  * d, dinv [n][ld] (permuted order). */

@@ -125,6 +125,8 @@ def lib() -> ctypes.CDLL:
                 ctypes.POINTER(_AdjDesc), i64, vp, i64, vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                 vp, vp]
         _LIB.oracle_num_threads.restype = ctypes.c_int
+        _LIB.oracle_set_num_threads.argtypes = [ctypes.c_int]
+        _LIB.oracle_set_num_threads.restype = None
     return _LIB
 
 
@@ -351,8 +353,9 @@ def num_threads() -> int:
 
 
 def set_threads(n: int) -> None:
-    os.environ["OMP_NUM_THREADS"] = str(int(n))
+    """Threads of the oracle's OpenMP loops from now on."""
+    lib().oracle_set_num_threads(int(n))
 
 
 __all__ = ["OracleAdjoint", "OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor",
-           "qdldl_solve"]
+           "qdldl_solve", "set_threads"]

@@ -141,6 +141,8 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_qdldl_plan_info": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
     "wmpc_qdldl_plan_slots": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64), _i32]),
     "wmpc_qdldl_plan_program": (ctypes.c_int, [_vp, _i32, _i32p, _i64, ctypes.POINTER(ctypes.c_int64)]),
+    "wmpc_set_instances_per_wave": (ctypes.c_int, [_i32]),
+    "wmpc_instances_per_wave": (ctypes.c_int32, [_i64]),
     "wmpc_qdldl_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i64, _vp]),
     "wmpc_qdldl_solve_f32": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
     "wmpc_qdldl_factor_solve_f32": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64,

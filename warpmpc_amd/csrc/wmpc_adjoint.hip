@@ -124,13 +124,13 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
     float* __restrict__ stash, float* __restrict__ rbuf, float* __restrict__ sol,
     float* __restrict__ lam, float* __restrict__ slk, float* __restrict__ dq,
     float* __restrict__ dl, float* __restrict__ du, float* __restrict__ dp,
-    float* __restrict__ da, int32_t refine, int64_t batch, int64_t ld) {
+    float* __restrict__ da, int32_t refine, int64_t batch, int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
-  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
-  // lanes past the batch mirror the last instance so that every lane takes
-  // part in the wave-shared staging of the sweeps
-  const int64_t b = b0 < batch ? b0 : batch - 1;
+  // every lane takes part in the wave-shared staging of the sweeps, lanes
+  // past the batch on the last instance
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
   float* mylds = lds + lane;
   const int64_t pes = pm_ld ? pm_ld : 1;
   const uint32_t pl = pm_ld ? (uint32_t)b : 0u;
@@ -449,10 +449,11 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
   if (lds > 64 * 1024)
     WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int64_t grid = (batch + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_osqp_adjoint, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream, P,
-                     dv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx, dinv, stash, rbuf,
-                     sol ? sol : solw, lam, slk, dq, dl, du, dp, da, refine, batch, ld);
+  const int32_t ipw = instances_per_wave(batch);
+  hipLaunchKernelGGL(k_osqp_adjoint, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
+                     (hipStream_t)stream, P, dv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx,
+                     dinv, stash, rbuf, sol ? sol : solw, lam, slk, dq, dl, du, dp, da, refine, batch,
+                     ld, ipw);
   return check_launch("osqp adjoint");
 }
 

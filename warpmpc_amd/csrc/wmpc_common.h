@@ -42,6 +42,14 @@ inline int check_launch(const char* what) {
   return WMPC_OK;
 }
 
+// Instances per wavefront of the batched kernels for a batch (8, 16, 32 or
+// 64): the override set by wmpc_set_instances_per_wave, else 64 once the batch
+// gives every SIMD of the chip a wavefront (kFillWaves wavefronts), else the
+// smallest power of two that still does.
+constexpr int64_t kFillWaves = 256 * 4;
+int32_t instances_per_wave(int64_t batch);
+inline int64_t wave_grid(int64_t batch, int32_t ipw) { return (batch + ipw - 1) / ipw; }
+
 // Largest dynamic LDS a workgroup may request on gfx950 (160 KiB per CU).
 constexpr size_t kMaxLdsBytes = 160 * 1024;
 // Default LDS budget of one 64-lane workgroup: four workgroups (one wavefront

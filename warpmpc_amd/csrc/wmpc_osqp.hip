@@ -121,13 +121,13 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
     const float* __restrict__ y0, float* __restrict__ lx, float* __restrict__ dinv, int64_t f_ld,
     float* __restrict__ w, float* __restrict__ rbuf, float* __restrict__ x, float* __restrict__ z,
     float* __restrict__ y, float* __restrict__ prim, float* __restrict__ dual,
-    float* __restrict__ obj, int32_t max_iter, int64_t batch, int64_t ld, int32_t lane_slots) {
+    float* __restrict__ obj, int32_t max_iter, int64_t batch, int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
-  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
-  // lanes past the batch mirror the last instance (identical loads and stores)
-  // so that every lane takes part in the wave-shared staging of the sweeps
-  const int64_t b = b0 < batch ? b0 : batch - 1;
+  // every lane takes part in the wave-shared staging of the sweeps, lanes
+  // past the batch on the last instance (identical loads and stores)
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
   float* mylds = lds + lane;
   const int64_t pb = pm_ld ? b : 0;
   const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
@@ -200,10 +200,11 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
 __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource src,
                                                        float* __restrict__ lx,
                                                        float* __restrict__ dinv, int64_t batch,
-                                                       int64_t ld) {
+                                                       int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
-  const int64_t b = (int64_t)blockIdx.x * kWave + threadIdx.x;
-  if (b >= batch) return;
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
+  if (b0 >= batch) return;
   const int64_t pb = src.pm_ld ? b : 0;
   factor_rows<false>(P, src, pb, Strided{lx, ld, b}, Strided{nullptr, 0, 0},
                      Strided{dinv, ld, b}, lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
@@ -397,10 +398,10 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
   const size_t lds = factor_lds(P);
   int rc = osqp_prepare_lds(k_osqp_factor, lds);
   if (rc) return rc;
-  const int64_t grid = (batch + kWave - 1) / kWave;
+  const int32_t ipw = instances_per_wave(batch);
   KktSource src{p_values, a_values, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-  hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream, P,
-                     src, lx, dinv, batch, ld);
+  hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
+                     (hipStream_t)stream, P, src, lx, dinv, batch, ld, ipw);
   return check_launch("osqp factor");
 }
 
@@ -425,7 +426,8 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
   const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
   const int64_t nk = P.n, nl = P.nnz_l;
-  const int64_t grid = (batch + kWave - 1) / kWave;
+  const int32_t ipw = instances_per_wave(batch);
+  const int64_t grid = wave_grid(batch, ipw);
   hipStream_t st = (hipStream_t)stream;
   float* stash = work;
   float* rbuf = work + nk * ld;
@@ -434,21 +436,20 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
     WMPC_REQUIRE(f_ld == 0 || f_ld >= batch, "f_ld must be 0 (shared) or >= batch");
     const bool sh = f_ld == 0;
     const size_t lds = admm_lds(P, sh, false);
-    const int32_t slots = admm_lane_slots(P, sh, false);
     if (sh) {
       int rc = osqp_prepare_lds(k_osqp_admm<true, false>, lds);
       if (rc) return rc;
       hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kWave), lds, st, P,
                          dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, (float*)lx,
                          (float*)dinv, (int64_t)0, stash, rbuf, x, z, y, prim, dual, obj, max_iter,
-                         batch, ld, slots);
+                         batch, ld, ipw);
     } else {
       int rc = osqp_prepare_lds(k_osqp_admm<false, false>, lds);
       if (rc) return rc;
       hipLaunchKernelGGL((k_osqp_admm<false, false>), dim3((unsigned)grid), dim3(kWave), lds, st,
                          P, dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, (float*)lx,
                          (float*)dinv, f_ld, stash, rbuf, x, z, y, prim, dual, obj, max_iter, batch,
-                         ld, slots);
+                         ld, ipw);
     }
     return check_launch("osqp admm");
   }
@@ -461,7 +462,7 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
     int rc = osqp_prepare_lds(k_osqp_factor, flds);
     if (rc) return rc;
     hipLaunchKernelGGL(k_osqp_factor, dim3(1), dim3(kWave), flds, st, P, src, lxw, dinvw,
-                       (int64_t)1, (int64_t)1);
+                       (int64_t)1, (int64_t)1, kWave);
     rc = check_launch("osqp shared factor");
     if (rc) return rc;
     const size_t lds = admm_lds(P, true, false);
@@ -469,8 +470,7 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
     if (rc) return rc;
     hipLaunchKernelGGL((k_osqp_admm<true, false>), dim3((unsigned)grid), dim3(kWave), lds, st, P,
                        dv, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, (int64_t)0,
-                       stash, rbuf, x, z, y, prim, dual, obj, max_iter, batch, ld,
-                       admm_lane_slots(P, true, false));
+                       stash, rbuf, x, z, y, prim, dual, obj, max_iter, batch, ld, ipw);
     return check_launch("osqp admm");
   }
   float* lxw = fwork;
@@ -480,7 +480,7 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
   if (rc) return rc;
   hipLaunchKernelGGL((k_osqp_admm<false, true>), dim3((unsigned)grid), dim3(kWave), lds, st, P, dv,
                      p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, ld, stash, rbuf, x,
-                     z, y, prim, dual, obj, max_iter, batch, ld, admm_lane_slots(P, false, true));
+                     z, y, prim, dual, obj, max_iter, batch, ld, ipw);
   return check_launch("osqp admm");
 }
 

@@ -6,6 +6,7 @@
 // one lane per instance, batch-major data, the working values of the current
 // rows in LDS (see wmpc_program.h), each factor/solve one launch.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -26,6 +27,15 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
+static std::atomic<int32_t> g_ipw{0};
+int32_t instances_per_wave(int64_t batch) {
+  const int32_t forced = g_ipw.load(std::memory_order_relaxed);
+  if (forced) return forced;
+  int32_t ipw = 8;
+  while (ipw < kWave && (batch + 2 * ipw - 1) / (2 * ipw) >= kFillWaves) ipw *= 2;
+  return ipw;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 
@@ -33,10 +43,11 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor(QdldlProg P, DirectSourc
                                                         float* __restrict__ lx,
                                                         float* __restrict__ d,
                                                         float* __restrict__ dinv, int64_t batch,
-                                                        int64_t ld) {
+                                                        int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
-  const int64_t b = (int64_t)blockIdx.x * kWave + threadIdx.x;
-  if (b >= batch) return;
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
+  if (b0 >= batch) return;
   factor_rows<false>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
                      lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
 }
@@ -44,13 +55,13 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor(QdldlProg P, DirectSourc
 __global__ __launch_bounds__(kWave) void k_qdldl_factor_solve(
     QdldlProg P, DirectSource src, const float* __restrict__ rhs, float* __restrict__ x,
     float* __restrict__ lx, float* __restrict__ d, float* __restrict__ dinv,
-    float* __restrict__ w, int64_t batch, int64_t ld, int32_t lane_slots) {
+    float* __restrict__ w, int64_t batch, int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
-  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
-  // lanes past the batch mirror the last instance (identical loads and stores)
-  // so that every lane takes part in the wave-shared staging
-  const int64_t b = b0 < batch ? b0 : batch - 1;
+  // every lane takes part in the wave-shared staging of the sweeps, lanes
+  // past the batch on the last instance
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
   const Strided stash{w, ld, b};
   factor_rows<true>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
                     lds + lane, ArrayRhs{CStrided{rhs, ld, b}}, stash);
@@ -65,11 +76,11 @@ __global__ __launch_bounds__(kWave) void k_qdldl_solve(QdldlProg P, const float*
                                                        int64_t f_ld, const float* __restrict__ rhs,
                                                        float* __restrict__ x,
                                                        float* __restrict__ w, int64_t batch,
-                                                       int64_t ld, int32_t lane_slots) {
+                                                       int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
-  const int64_t b0 = (int64_t)blockIdx.x * kWave + lane;
-  const int64_t b = b0 < batch ? b0 : batch - 1;
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
   const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
   // right-hand side in first-touch order, staged in the x rows
 #pragma unroll 8
@@ -345,9 +356,10 @@ int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, fl
   const size_t lds = lds_bytes_factor(plan->prog, false);
   int rc = prepare_lds(k_qdldl_factor, lds);
   if (rc) return rc;
-  const int64_t grid = (batch + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_qdldl_factor, dim3((unsigned)grid), dim3(kWave), lds, (hipStream_t)stream,
-                     plan->prog, DirectSource{a_values, ld}, lx, d, dinv, batch, ld);
+  const int32_t ipw = instances_per_wave(batch);
+  hipLaunchKernelGGL(k_qdldl_factor, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
+                     (hipStream_t)stream, plan->prog, DirectSource{a_values, ld}, lx, d, dinv, batch,
+                     ld, ipw);
   return check_launch("qdldl factor");
 }
 
@@ -365,21 +377,21 @@ int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t l
   WMPC_REQUIRE(work_len >= (int64_t)plan->prog.n * ld, "work must hold n * ld floats");
   const bool shared = lx_ld == 0;
   const size_t lds = lds_bytes_solve(plan->prog, shared);
-  const int32_t slots = lane_slots_solve(plan->prog, shared);
-  const int64_t grid = (batch + kWave - 1) / kWave;
+  const int32_t ipw = instances_per_wave(batch);
+  const int64_t grid = wave_grid(batch, ipw);
   WMPC_REQUIRE(x != rhs, "x must not alias rhs (x stages the right-hand side)");
   if (shared) {
     int rc = prepare_lds(k_qdldl_solve<true>, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(k_qdldl_solve<true>, dim3((unsigned)grid), dim3(kWave), lds,
                        (hipStream_t)stream, plan->prog, lx, dinv, (int64_t)0, rhs, x, work, batch,
-                       ld, slots);
+                       ld, ipw);
   } else {
     int rc = prepare_lds(k_qdldl_solve<false>, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(k_qdldl_solve<false>, dim3((unsigned)grid), dim3(kWave), lds,
                        (hipStream_t)stream, plan->prog, lx, dinv, lx_ld, rhs, x, work, batch, ld,
-                       slots);
+                       ipw);
   }
   return check_launch("qdldl solve");
 }
@@ -398,11 +410,21 @@ int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_valu
   const size_t lds = lds_bytes_factor(plan->prog, true);
   int rc = prepare_lds(k_qdldl_factor_solve, lds);
   if (rc) return rc;
-  const int64_t grid = (batch + kWave - 1) / kWave;
-  hipLaunchKernelGGL(k_qdldl_factor_solve, dim3((unsigned)grid), dim3(kWave), lds,
+  const int32_t ipw = instances_per_wave(batch);
+  hipLaunchKernelGGL(k_qdldl_factor_solve, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, plan->prog, DirectSource{a_values, ld}, rhs, x, lx, d,
-                     dinv, work, batch, ld, lane_slots_factor(plan->prog, true));
+                     dinv, work, batch, ld, ipw);
   return check_launch("qdldl factor+solve");
 }
+
+int wmpc_set_instances_per_wave(int32_t ipw) {
+  clear_error();
+  WMPC_REQUIRE(ipw == 0 || ipw == 8 || ipw == 16 || ipw == 32 || ipw == 64,
+               "instances per wavefront must be 0 (automatic), 8, 16, 32 or 64");
+  g_ipw.store(ipw, std::memory_order_relaxed);
+  return WMPC_OK;
+}
+
+int32_t wmpc_instances_per_wave(int64_t batch) { return batch < 0 ? -1 : instances_per_wave(batch); }
 
 }  // extern "C"

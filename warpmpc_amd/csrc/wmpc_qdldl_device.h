@@ -19,6 +19,18 @@ namespace wmpc {
 
 constexpr int kWave = 64;
 
+// Instance of this lane.  A wavefront serves `ipw` instances (a power of two,
+// 8..64): lane l works on instance blockIdx.x * ipw + (l mod ipw), so with
+// ipw < 64 the 64 / ipw lanes of one instance do identical work (identical
+// loads, identical values stored to identical addresses).  Fewer instances per
+// wavefront give a small batch more wavefronts, one per SIMD.  Lanes past the
+// batch take the last instance.  *b0 receives the unclamped index.
+__device__ __forceinline__ int64_t lane_instance(int32_t ipw, int64_t batch, int64_t* b0) {
+  const int64_t i = (int64_t)blockIdx.x * ipw + (int64_t)(threadIdx.x & (ipw - 1));
+  *b0 = i;
+  return i < batch ? i : batch - 1;
+}
+
 // Device view of one pattern's programs (see wmpc_program.h for their meaning).
 struct QdldlProg {
   int32_t n = 0, nnz_a = 0, nnz_l = 0;
