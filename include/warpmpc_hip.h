@@ -90,7 +90,7 @@ int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t*
                             int64_t* len);
 
 /* Instances per 64-lane wavefront of every batched kernel: 0 = automatic (64
- * once the batch gives each SIMD of the chip a wavefront, i.e. batch >= 65536;
+ * once the batch gives each of the 256 CUs a wavefront, i.e. batch >= 16384;
  * below that the smallest of 8, 16, 32 that still does, the lanes of one
  * instance doing identical work), or 8, 16, 32, 64.  Process-wide.  No
  * counterpart in the reference (its Warp kernels use one thread per instance):

@@ -50,7 +50,8 @@ def _same(outs):
 def test_auto_setting_and_bad_values(gpu):
     lib = _lib.load()
     assert lib.wmpc_instances_per_wave(65536) == 64
-    assert lib.wmpc_instances_per_wave(16384) == 16
+    assert lib.wmpc_instances_per_wave(16384) == 64
+    assert lib.wmpc_instances_per_wave(4096) == 16
     assert lib.wmpc_instances_per_wave(100) == 8
     assert lib.wmpc_set_instances_per_wave(12) != 0
     assert lib.wmpc_set_instances_per_wave(0) == 0

@@ -44,9 +44,11 @@ inline int check_launch(const char* what) {
 
 // Instances per wavefront of the batched kernels for a batch (8, 16, 32 or
 // 64): the override set by wmpc_set_instances_per_wave, else 64 once the batch
-// gives every SIMD of the chip a wavefront (kFillWaves wavefronts), else the
-// smallest power of two that still does.
-constexpr int64_t kFillWaves = 256 * 4;
+// gives every CU of the chip a wavefront (kFillWaves wavefronts), else the
+// smallest power of two that still does.  The kernels are bound by the scalar
+// issue of a CU (shared by its four SIMDs), so more wavefronts per CU than one
+// only add scalar work per instance; more CUs add scalar units.
+constexpr int64_t kFillWaves = 256;
 int32_t instances_per_wave(int64_t batch);
 inline int64_t wave_grid(int64_t batch, int32_t ipw) { return (batch + ipw - 1) / ipw; }
 
