@@ -1,87 +1,90 @@
-"""Segmentation known answers of tests/test_jax_qdldl_segments.py:56-88 (CPU only)."""
+"""Segment planner (the ``segmented`` option), CPU only.
+
+Pins: the known answers of the reference's segment tests
+(tests/test_jax_qdldl_segments.py:56-88: row widths [100, 1, 1, 100] in three
+segments cost 400 padded slots greedily and 202 optimally), exhaustive search
+over every split for n <= 8, and the partition invariants every strategy must
+keep.
+"""
 
 from __future__ import annotations
 
 import itertools
 
 import numpy as np
+import pytest
 
-from warpmpc_amd.qdldl.segments import (
-    make_fixed_segments,
-    make_greedy_segments,
-    make_level_schedule,
-    make_optimal_segments,
-    make_segments,
-    make_solve_levels,
-)
+from warpmpc_amd.qdldl.segments import make_segments
+
+STRATEGIES = ("fixed", "greedy", "optimal")
 
 
-def _mask(counts):
-    width = max(counts, default=0)
-    mask = np.zeros((len(counts), width), dtype=bool)
-    for i, c in enumerate(counts):
-        mask[i, :c] = True
+def rows_of_width(widths):
+    """Row mask whose row i holds widths[i] entries."""
+    widths = list(widths)
+    mask = np.zeros((len(widths), max(widths, default=0)), dtype=bool)
+    for i, w in enumerate(widths):
+        mask[i, :w] = True
     return mask
 
 
-def _cost(counts, segments):
-    return sum((int(c[-1]) + 1 - int(c[0])) * int(np.max(counts[int(c[0]) : int(c[-1]) + 1], initial=0))
-               for _, c in segments)
+def bounds(segments):
+    return [(int(cols[0]), int(cols[-1]) + 1) for _, cols in segments]
 
 
-def _intervals(segments):
-    return [(int(c[0]), int(c[-1]) + 1) for _, c in segments]
+def padded_cost(widths, segments):
+    widths = np.asarray(widths)
+    return sum((b - a) * int(widths[a:b].max(initial=0)) for a, b in bounds(segments))
 
 
-def _brute(counts, k):
-    n = len(counts)
-    best = None
-    for splits in itertools.combinations(range(1, n), k - 1):
-        starts = (0, *splits)
-        stops = (*splits, n)
-        cost = sum((b - a) * int(np.max(counts[a:b], initial=0)) for a, b in zip(starts, stops))
-        best = cost if best is None else min(best, cost)
-    return best
+def cheapest_split(widths, pieces):
+    n = len(widths)
+    costs = []
+    for cuts in itertools.combinations(range(1, n), pieces - 1):
+        edges = (0, *cuts, n)
+        costs.append(sum((b - a) * max(widths[a:b], default=0) for a, b in zip(edges, edges[1:])))
+    return min(costs)
 
 
-def test_strategy_selector_uses_segment_budget():
-    counts = np.array([5, 1, 4, 1, 3])
-    mask = _mask(counts.tolist())
-    fixed = make_segments(mask, segment_budget=3, segment_strategy="fixed")
-    greedy = make_segments(mask, segment_budget=3, segment_strategy="greedy")
-    optimal = make_segments(mask, segment_budget=3, segment_strategy="optimal")
-    assert _intervals(fixed) == _intervals(make_fixed_segments(mask, 3))
-    assert _intervals(greedy) == _intervals(make_greedy_segments(mask, 3))
-    assert _intervals(optimal) == _intervals(make_optimal_segments(mask, 3))
-    assert len(fixed) == 3 and len(greedy) <= 3 and len(optimal) == 3
+@pytest.mark.parametrize("strategy, expected", [("greedy", 400), ("optimal", 202)])
+def test_reference_known_answer(strategy, expected):
+    widths = [100, 1, 1, 100]
+    segs = make_segments(rows_of_width(widths), segment_budget=3, segment_strategy=strategy)
+    assert padded_cost(widths, segs) == expected
 
 
-def test_optimal_segments_can_cross_zero_saving_splits():
-    counts = np.array([100, 1, 1, 100])
-    mask = _mask(counts.tolist())
-    assert _cost(counts, make_greedy_segments(mask, 3)) == 400
-    optimal = make_optimal_segments(mask, 3)
-    assert len(optimal) == 3
-    assert _cost(counts, optimal) == 202
+@pytest.mark.parametrize("n", range(1, 9))
+def test_optimal_equals_exhaustive_search(n):
+    rng = np.random.default_rng(100 + n)
+    for _ in range(6):
+        widths = rng.integers(0, 9, size=n).tolist()
+        for pieces in range(1, n + 1):
+            segs = make_segments(rows_of_width(widths), segment_budget=pieces, segment_strategy="optimal")
+            assert len(segs) == pieces
+            assert padded_cost(widths, segs) == cheapest_split(widths, pieces)
 
 
-def test_optimal_segments_match_bruteforce_for_small_inputs():
-    rng = np.random.default_rng(0)
-    for n in range(1, 9):
-        for _ in range(8):
-            counts = rng.integers(0, 8, size=n)
-            mask = _mask(counts.tolist())
-            for k in range(1, n + 1):
-                seg = make_optimal_segments(mask, k)
-                assert len(seg) == k
-                assert _cost(counts, seg) == _brute(counts, k)
+@pytest.mark.parametrize("strategy", STRATEGIES)
+@pytest.mark.parametrize("budget", [1, 2, 5, 64])
+def test_segments_partition_the_rows(strategy, budget):
+    widths = np.random.default_rng(budget).integers(0, 20, size=37).tolist()
+    segs = make_segments(rows_of_width(widths), segment_budget=budget, segment_strategy=strategy)
+    edges = bounds(segs)
+    assert edges[0][0] == 0 and edges[-1][1] == len(widths)
+    assert all(a < b for a, b in edges)
+    assert all(edges[i][1] == edges[i + 1][0] for i in range(len(edges) - 1))
+    assert len(segs) <= min(budget, len(widths))
+    for start, cols in segs:
+        assert start == cols[0] and np.array_equal(cols, np.arange(cols[0], cols[-1] + 1))
+    # no strategy can do better than the exact minimum, nor worse than one segment
+    best = padded_cost(widths, make_segments(rows_of_width(widths), segment_budget=budget,
+                                             segment_strategy="optimal"))
+    assert best <= padded_cost(widths, segs) <= len(widths) * max(widths)
 
 
-def test_solve_levels_and_schedule():
-    # chain 0 -> 1 -> 2 plus independent 3
-    col_rows = np.array([[1], [2], [0], [0]])
-    col_mask = np.array([[True], [True], [False], [False]])
-    levels = make_solve_levels(col_rows, col_mask)
-    assert [lv.tolist() for lv in levels] == [[0, 3], [1], [2]]
-    sched = make_level_schedule(levels, serial_width_threshold=1)
-    assert sched == (("parallel", 0, 1, 0, 2), ("serial", 1, 3, 2, 0))
+def test_empty_pattern_and_bad_options():
+    assert make_segments(rows_of_width([]), segment_budget=4, segment_strategy="optimal") == []
+    with pytest.raises(ValueError):
+        make_segments(rows_of_width([1, 2]), segment_budget=0, segment_strategy="greedy")
+    with pytest.raises(ValueError):
+        make_segments(rows_of_width([1, 2]), segment_budget=2, segment_strategy="balanced")

@@ -10,12 +10,13 @@ Arrays are PyTorch-ROCm tensors on the current GPU; numpy input is copied there.
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 
 from .. import _lib
 from .._device import as_device_f32, ptr, require_gpu, stream_ptr, torch
-from .segments import make_level_schedule, make_segments, make_solve_levels
+from .segments import make_segments
 from .tables import ragged_program
 from .types import CompiledQDLDL, QDLDLPlan
 
@@ -60,18 +61,28 @@ class DevicePlan:
             self.handle = None
 
 
-def _variant_name(*, transpose_work, segmented, segment_budget, segment_strategy,
-                  level_scheduled_solve, level_scheduled_solve_threshold) -> str:
-    parts = ["hip"]
-    if transpose_work:
-        parts.append("transpose")
-    if segmented:
-        parts += ["segmented", f"budget{segment_budget}", segment_strategy]
-    if level_scheduled_solve:
-        parts.append("levelsolve")
-        if level_scheduled_solve_threshold != 1:
-            parts.append(f"threshold{level_scheduled_solve_threshold}")
-    return "+".join(parts)
+# The kernels that run: unpadded per-row programs, one lane per instance,
+# batch-major ("transpose") work arrays.
+VARIANT = "hip+transpose"
+
+_warned_options: set[str] = set()
+
+
+def _note_options_without_effect(segmented: bool, level_scheduled_solve: bool) -> dict:
+    """``segmented`` and ``level_scheduled_solve`` choose execution strategies of
+    the Warp kernels (``warp_backend.py:1129-1149, 1098-1126``): narrower padding
+    of the per-thread loops, and level-parallel triangular solves with several
+    threads per instance.  They never change results.  The HIP kernels walk
+    unpadded programs and run each solve inside its instance's lane, so there is
+    nothing for these options to select: they are validated, recorded as
+    without effect and reported once per process."""
+    flags = {"segmented": bool(segmented), "level_scheduled_solve": bool(level_scheduled_solve)}
+    for name, on in flags.items():
+        if on and name not in _warned_options:
+            _warned_options.add(name)
+            warnings.warn(f"{name}=True has no effect on the HIP QDLDL backend (its kernels have no padded loops "
+                          "and solve each instance in one lane)", UserWarning, stacklevel=3)
+    return {k: v for k, v in flags.items() if v}
 
 
 def compile_qdldl_variant(
@@ -92,8 +103,9 @@ def compile_qdldl_variant(
 
     Like the Warp backend, the HIP backend works in the symbolic layout and
     requires ``transpose_work=True`` and float32 (``warp_backend.py:1197-1228``).
-    The segment and level options are validated and recorded in ``variant``; the
-    HIP kernels walk unpadded row programs, so they need no padding segments.
+    The segment and level options are validated like the reference validates
+    them and then have no effect (``_note_options_without_effect``); ``variant``
+    names the kernels that run.
     """
     dtype = np.dtype(dtype)
     factor_backend = normalize_backend_name(backend if factor_backend is None else factor_backend, role="factor")
@@ -107,8 +119,7 @@ def compile_qdldl_variant(
     if segmented:
         make_segments(plan.process_mask, segment_budget=segment_budget, segment_strategy=segment_strategy)
         make_segments(plan.col_mask, segment_budget=segment_budget, segment_strategy=segment_strategy)
-    levels = make_solve_levels(plan.col_rows, plan.col_mask)
-    schedule = make_level_schedule(levels, serial_width_threshold=level_scheduled_solve_threshold)
+    without_effect = _note_options_without_effect(segmented, level_scheduled_solve)
     require_gpu()
     dev = DevicePlan(plan)
     lib = _lib.load()
@@ -172,20 +183,13 @@ def compile_qdldl_variant(
         factor=factor,
         solve=solve,
         factor_and_solve=factor_and_solve,
-        variant=_variant_name(
-            transpose_work=transpose_work,
-            segmented=segmented,
-            segment_budget=segment_budget,
-            segment_strategy=segment_strategy,
-            level_scheduled_solve=level_scheduled_solve,
-            level_scheduled_solve_threshold=level_scheduled_solve_threshold,
-        ),
+        variant=VARIANT,
         values_layout="original_symbolic",
         rhs_layout="symbolic",
         backend="hip",
         factor_backend=factor_backend,
         solve_backend=solve_backend,
-        extras={"device_plan": dev, "solve_levels": levels, "level_schedule": schedule},
+        extras={"device_plan": dev, "options_without_effect": without_effect},
     )
 
 

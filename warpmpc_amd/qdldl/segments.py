@@ -1,17 +1,16 @@
-"""Segmentation of row/column scans and dependency levels of the triangular solve.
+"""Segmentation of the row/column scans (the ``segmented`` option).
 
-Restates ``warpmpc/jax_qdldl/core.py:320-510``:
+Restates ``warpmpc/jax_qdldl/core.py:320-485``:
 
 * ``fixed`` segments: ``budget`` nearly equal contiguous pieces;
 * ``greedy`` segments: repeatedly apply the split with the largest saving of
   padded work ``(stop - start) * max(counts[start:stop])``;
 * ``optimal`` segments: exact minimum of the total padded work over all
-  contiguous splits into ``min(budget, n)`` pieces, by dynamic programming;
-* solve levels: level of a column = longest path to it in L's DAG.
+  contiguous splits into ``min(budget, n)`` pieces, by dynamic programming.
 
-The HIP kernels walk ragged (unpadded) tables, so segmentation does not change
-their arithmetic; these routines decide the reference variant names, feed the
-level-scheduled schedule and are checked against the reference's known answers.
+The HIP kernels walk ragged (unpadded) tables, so segments do not change what
+they run; ``compile_qdldl_variant`` uses these routines to validate the
+``segmented`` options as the reference does.
 """
 
 from __future__ import annotations
@@ -138,43 +137,9 @@ def make_segments(mask: np.ndarray, *, segment_budget: int, segment_strategy: st
     raise ValueError(f"unknown segment_strategy {segment_strategy!r}")
 
 
-def make_solve_levels(col_rows: np.ndarray, col_mask: np.ndarray) -> list[np.ndarray]:
-    """Columns grouped by dependency level (``core.py:488-510``)."""
-    n = int(col_rows.shape[0])
-    level = np.zeros(n, dtype=np.int64)
-    for c in range(n):
-        rows = col_rows[c][np.asarray(col_mask[c], dtype=bool)]
-        if rows.size:
-            level[rows] = np.maximum(level[rows], level[c] + 1)
-    top = int(level.max(initial=0))
-    return [np.flatnonzero(level == v).astype(np.int32) for v in range(top + 1)]
-
-
-def make_level_schedule(levels: list[np.ndarray], *, serial_width_threshold: int):
-    """Runs of narrow levels become one serial step (``warp_backend.py:1098-1126``)."""
-    offsets = np.zeros(len(levels) + 1, dtype=np.int64)
-    offsets[1:] = np.cumsum([len(v) for v in levels])
-    out = []
-    i = 0
-    while i < len(levels):
-        width = len(levels[i])
-        if width <= serial_width_threshold:
-            j = i + 1
-            while j < len(levels) and len(levels[j]) <= serial_width_threshold:
-                j += 1
-            out.append(("serial", i, j, int(offsets[i]), 0))
-            i = j
-        else:
-            out.append(("parallel", i, i + 1, int(offsets[i]), width))
-            i += 1
-    return tuple(out)
-
-
 __all__ = [
     "make_fixed_segments",
     "make_greedy_segments",
-    "make_level_schedule",
     "make_optimal_segments",
     "make_segments",
-    "make_solve_levels",
 ]
