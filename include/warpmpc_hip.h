@@ -199,6 +199,27 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream);
 
 /*
+ * wmpc_osqp_solve_f32 without a work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * scratch is a fixed device array inside the library, one per GPU, of
+ * wmpc_osqp_noalloc_max_floats() floats (64 MiB), held until the process ends;
+ * a call whose wmpc_osqp_work_floats(plan, mode, ld) exceeds it fails with
+ * WMPC_ERR_INVALID.  The call allocates nothing and needs no part of the
+ * scratch cleared (every word it reads it wrote earlier in the same call).
+ * All calls on one GPU share that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller.
+ * Other arguments, outputs and errors as wmpc_osqp_solve_f32.
+ */
+int64_t wmpc_osqp_noalloc_max_floats(void);
+int wmpc_osqp_solve_f32_noalloc(const wmpc_osqp_plan* plan, const float* p_values,
+                                const float* a_values, int64_t pm_ld, const float* q,
+                                const float* l, const float* u, const float* x0, const float* z0,
+                                const float* y0, const float* lx, const float* dinv, int64_t f_ld,
+                                float* x, float* z, float* y, float* prim, float* dual, float* obj,
+                                int32_t max_iter, int64_t batch, int64_t ld, void* stream);
+
+/*
  * OSQP adjoint derivative (warpmpc/jax_osqp/solver.py:1035-1192).  The
  * descriptor carries the OSQPDerivativePlan fields (types.py:21-55, built by
  * solver.py:413-571) and the A entry coordinates of the FixedOSQPPlan.  With

@@ -154,6 +154,10 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_osqp_solve_f32": (ctypes.c_int, [
         _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
+    "wmpc_osqp_noalloc_max_floats": (ctypes.c_int64, []),
+    "wmpc_osqp_solve_f32_noalloc": (ctypes.c_int, [
+        _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+        _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
     "wmpc_adjoint_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointDesc), _vp, ctypes.POINTER(_vp)]),
     "wmpc_adjoint_plan_destroy": (ctypes.c_int, [_vp]),
     "wmpc_adjoint_dim": (ctypes.c_int32, [_vp]),

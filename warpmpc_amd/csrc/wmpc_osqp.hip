@@ -13,6 +13,7 @@
 // first-touch order.  Scratch: stash and right-hand side, [n_kkt][ld] each.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -405,13 +406,18 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
   return check_launch("osqp factor");
 }
 
-int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
-                        const float* a_values, int64_t pm_ld, const float* q, const float* l,
-                        const float* u, const float* x0, const float* z0, const float* y0,
-                        const float* lx, const float* dinv, int64_t f_ld, float* work,
-                        int64_t work_len, float* x, float* z, float* y, float* prim, float* dual,
-                        float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream) {
-  clear_error();
+}  // extern "C"
+
+namespace wmpc {
+
+// Body of wmpc_osqp_solve_f32; `work_what` names the scratch in size errors.
+static int osqp_solve(const wmpc_osqp_plan* plan, const float* p_values, const float* a_values,
+                      int64_t pm_ld, const float* q, const float* l, const float* u,
+                      const float* x0, const float* z0, const float* y0, const float* lx,
+                      const float* dinv, int64_t f_ld, float* work, int64_t work_len,
+                      const char* work_what, float* x, float* z, float* y, float* prim,
+                      float* dual, float* obj, int32_t max_iter, int64_t batch, int64_t ld,
+                      void* stream) {
   WMPC_REQUIRE(plan && (batch == 0 || (p_values && a_values && q && l && u && work && x && z && y &&
                                        prim && dual && obj)),
                "null argument");
@@ -421,8 +427,7 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
   if (batch == 0) return WMPC_OK;
   WMPC_REQUIRE((lx == nullptr) == (dinv == nullptr), "lx and dinv must be given together");
   const int32_t mode = lx ? 0 : (pm_ld == 0 ? 1 : 2);
-  WMPC_REQUIRE(work_len >= wmpc_osqp_work_floats(plan, mode, ld),
-               "work is smaller than wmpc_osqp_work_floats(plan, mode, ld)");
+  WMPC_REQUIRE(work_len >= wmpc_osqp_work_floats(plan, mode, ld), work_what);
   const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
   const int64_t nk = P.n, nl = P.nnz_l;
@@ -482,6 +487,77 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                      p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, ld, stash, rbuf, x,
                      z, y, prim, dual, obj, max_iter, batch, ld, ipw);
   return check_launch("osqp admm");
+}
+
+}  // namespace wmpc
+
+// Scratch of wmpc_osqp_solve_f32_noalloc: a fixed device array, one copy per
+// GPU the library runs on, held until the process ends.  Host code never uses
+// this name as an address (on the host it is only a stand-in for the symbol):
+// the device address comes from hipGetSymbolAddress, once per device.
+static constexpr int64_t kNoAllocScratchFloats = int64_t(16) << 20;  // 64 MiB
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace wmpc {
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on first use).
+static int scratch_of_current_device(float** out) {
+  int dev = 0;
+  WMPC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    set_error("device index beyond the scratch table of the no-allocation entry point");
+    return WMPC_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> guard(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    WMPC_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return WMPC_OK;
+}
+
+}  // namespace wmpc
+
+extern "C" {
+
+int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
+                        const float* a_values, int64_t pm_ld, const float* q, const float* l,
+                        const float* u, const float* x0, const float* z0, const float* y0,
+                        const float* lx, const float* dinv, int64_t f_ld, float* work,
+                        int64_t work_len, float* x, float* z, float* y, float* prim, float* dual,
+                        float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream) {
+  clear_error();
+  return osqp_solve(plan, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lx, dinv, f_ld, work,
+                    work_len, "work is smaller than wmpc_osqp_work_floats(plan, mode, ld)", x, z, y,
+                    prim, dual, obj, max_iter, batch, ld, stream);
+}
+
+int64_t wmpc_osqp_noalloc_max_floats(void) { return kNoAllocScratchFloats; }
+
+int wmpc_osqp_solve_f32_noalloc(const wmpc_osqp_plan* plan, const float* p_values,
+                                const float* a_values, int64_t pm_ld, const float* q,
+                                const float* l, const float* u, const float* x0, const float* z0,
+                                const float* y0, const float* lx, const float* dinv, int64_t f_ld,
+                                float* x, float* z, float* y, float* prim, float* dual, float* obj,
+                                int32_t max_iter, int64_t batch, int64_t ld, void* stream) {
+  clear_error();
+  if (batch == 0) {
+    WMPC_REQUIRE(plan, "null argument");
+    return WMPC_OK;
+  }
+  float* work = nullptr;
+  if (int rc = scratch_of_current_device(&work)) return rc;
+  return osqp_solve(plan, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lx, dinv, f_ld, work,
+                    kNoAllocScratchFloats,
+                    "batch too large for the no-allocation entry point: "
+                    "wmpc_osqp_work_floats(plan, mode, ld) exceeds wmpc_osqp_noalloc_max_floats()",
+                    x, z, y, prim, dual, obj, max_iter, batch, ld, stream);
 }
 
 }  // extern "C"
