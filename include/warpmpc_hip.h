@@ -79,7 +79,10 @@ int wmpc_qdldl_plan_info(const wmpc_qdldl_plan* plan, int32_t* n, int32_t* nnz_a
 /* Slot counts of a plan's programs, out[0..8] = reach slots, column-window
  * slots, live-L slots in LDS, backward-window slots, live-L slots needed to keep
  * every re-read on chip, L re-reads served from global memory, LDS bytes of the
- * fused factor+solve, LDS bytes of the solve, LDS budget.  len >= 9. */
+ * fused factor+solve, LDS bytes of the solve, LDS budget; out[9] (len >= 10)
+ * = LDS columns of the windows: 64, or 32/16/8 for a pattern whose windows do
+ * not fit in a workgroup's LDS otherwise (launches then serve at most that many
+ * instances per wavefront).  len >= 9. */
 int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len);
 /* Host copy of one program table (which: 0 row, 1 init, 2 proc, 3 upd, 4 bcol,
  * 5 bent, 6 fcol, 7 bcol2, 8 fent, 9 ftpos, 10 frec, 11 brec); *len receives

@@ -2,12 +2,13 @@
 
 The HIP library derives, per pattern, programs whose operands are LDS slots
 (reach slots Y, column window CW/FW, live-L slots AL, backward window BW) or L
-entries re-read from global memory.  This module replays exactly those
+entries re-read from global memory.  This module replays those
 programs for one instance in float64 with one Python dict per LDS region, so a
 slot that is reused while still live, or read before it is written, changes
-the result and fails the comparison with the oracle.  It mirrors
-``factor_rows``, ``forward_rows`` and ``backward_cols`` of
-``warpmpc_amd/csrc/wmpc_qdldl_device.h``.
+the result and fails the comparison with the oracle.  ``replay_factor_solve``
+mirrors ``factor_rows`` with the fused forward substitution
+(``wmpc_qdldl_device.h``); ``replay_record_sweeps`` mirrors the chunked
+``fwd_cols`` / ``bwd_cols`` of ``wmpc_stream.h``.
 """
 
 from __future__ import annotations
@@ -81,57 +82,6 @@ class _Region:
 
 
 FIRST_TOUCH = 1 << 16
-
-
-def replay_column_sweeps(hp: HostPlan, lx, dinv, rhs_orig):
-    """``fwd_cols`` + ``bwd_cols`` of ``wmpc_stream.h`` for one instance.
-
-    The right-hand side is staged in first-touch order (``R[ftpos[c]] =
-    rhs[p[c]]``), the forward sweep scatters in CSC order and stashes the
-    D⁻¹-scaled result, the backward sweep gathers in descending CSC order.
-    """
-    n, nl = hp.plan.n, hp.plan.nnz_l
-    p = hp.tables["p"]
-    R = np.zeros(n)
-    R[hp.ftpos] = rhs_orig[p]
-    W = _Region("W", hp.slots["backwin"])
-    stash = np.zeros(n)
-    q = ir = 0
-    for c in range(n):
-        ws, qend = int(hp.fcol[c][0]), int(hp.fcol[c][1])
-        if ws >= 0:
-            val = W[ws]
-        else:
-            val = R[ir]
-            ir += 1
-        stash[c] = val * dinv[c]
-        while q < qend:
-            e = int(hp.fent[q])
-            slot = e & (FIRST_TOUCH - 1)
-            if e & FIRST_TOUCH:
-                W[slot] = R[ir] - lx[q] * val
-                ir += 1
-            else:
-                W[slot] = W[slot] - lx[q] * val
-            q += 1
-    assert ir == n, "first-touch stream not fully consumed"
-    x = np.zeros(n)
-    W = _Region("W", hp.slots["backwin"])
-    i = 0
-    for k in range(n):
-        ws, qbeg, pc = (int(v) for v in hp.bcol2[k][:3])
-        corr = 0.0
-        while nl - 1 - i >= qbeg:
-            qq = nl - 1 - i
-            corr += lx[qq] * W[int(hp.fent[qq]) & (FIRST_TOUCH - 1)]
-            i += 1
-        w = stash[n - 1 - k] - corr
-        if ws >= 0:
-            W[ws] = w
-        x[pc] = w
-    return x
-
-
 MARK = 1 << 20
 NO_SLOT = 0xFFFF
 V_PER_CHUNK = 16
@@ -310,27 +260,3 @@ def replay_backward(hp: HostPlan, lx, dinv, stash):
             BW[bs] = acc
         x[p[c]] = acc
     return x
-
-
-def replay_solve(hp: HostPlan, lx, dinv, rhs):
-    """forward_rows + backward_cols for one instance with a given factor."""
-    n = hp.plan.n
-    p = hp.tables["p"]
-    FW = _Region("FW", hp.slots["colwin"])
-    stash = np.zeros(n)
-    r = 0
-    for k in range(n):
-        r1, _, ck, _ = (int(v) for v in hp.row[k])
-        acc = rhs[p[k]]
-        while r < r1:
-            cs, lidx = int(hp.proc[r][0]), int(hp.proc[r][1])
-            acc -= lx[lidx] * FW[cs]
-            r += 1
-        if ck >= 0:
-            FW[ck] = acc
-        stash[k] = acc
-    return replay_backward(hp, lx, dinv, stash)
-
-
-__all__ = ["HostPlan", "replay_backward", "replay_column_sweeps", "replay_factor_solve", "replay_record_sweeps",
-           "replay_solve"]

@@ -121,3 +121,109 @@ def test_minimal_osqp_example(gpu):
     out = solver.solve(pv, av, q, lb, ub)
     ref = oracle.OracleOSQP(plan).solve(pv, av, q, lb, ub)
     _check(out, ref)
+
+
+def _dense_row_problem():
+    """Mass chain N=10 plus one constraint on the sum of all variables: its KKT
+    column is dense, so the factor windows need the 32-column LDS layout."""
+    prob = mass_chain_mpc(horizon=10)
+    a = sp.vstack([prob.a_matrix, sp.csc_matrix(np.ones((1, prob.n)))], format="csc")
+    l = np.append(prob.l, -5.0)
+    u = np.append(prob.u, 5.0)
+    return prob, a, l, u
+
+
+def test_dense_coupling_row_matches_oracle(gpu):
+    from warpmpc_amd import _lib
+
+    prob, a, l, u = _dense_row_problem()
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, a, l, u, s)
+    solver = compile_osqp(plan, np.float32)
+    slots = _lib.plan_slots(solver.extras["device_plan"].kkt.handle)
+    assert slots["lds_columns"] == 32 and slots["global_rereads"] > 0
+    b = 300
+    rng = np.random.default_rng(8)
+    x0 = rng.uniform(-1, 1, size=(b, 12))
+    lb, ub = np.tile(l, (b, 1)), np.tile(u, (b, 1))
+    lb[:, :12] = -x0
+    ub[:, :12] = -x0
+    pv, av = plan.p_upper.data[None], plan.a_matrix.data[None]
+    q = np.zeros((b, plan.n))
+    out = solver.solve(pv, av, q, lb, ub)
+    _check(out, oracle.OracleOSQP(plan).solve(pv, av, q, lb, ub))
+    pvi = pv * rng.uniform(0.5, 2.0, size=(b, plan.nnz_p))
+    avi = np.broadcast_to(av, (b, plan.nnz_a))
+    out = solver.solve(pvi, avi, q, lb, ub)
+    _check(out, oracle.OracleOSQP(plan).solve(pvi, avi, q, lb, ub))
+
+
+def test_per_instance_factor_reuse_and_warm_start(gpu, chain):
+    prob, plan, solver = chain
+    b = 131
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=6)
+    rng = np.random.default_rng(6)
+    pv = pv * rng.uniform(0.5, 2.0, size=(b, pv.shape[1]))
+    av = np.broadcast_to(av, (b, av.shape[1])).copy()
+    lx, dinv = solver.factor(pv, av)
+    assert lx.shape == (plan.qdldl_plan.nnz_l, b)
+    out = solver.solve_with_factor(lx, dinv, pv, av, q, l, u)
+    _check(out, oracle.OracleOSQP(plan).solve(pv, av, q, l, u))
+    ws = (rng.normal(size=(b, plan.n)), rng.normal(size=(b, plan.m)), rng.normal(size=(b, plan.m)))
+    out = solver.solve_with_factor_warm_start(lx, dinv, pv, av, q, l, u, *ws)
+    _check(out, oracle.OracleOSQP(plan).solve(pv, av, q, l, u, x0=ws[0], z0=ws[1], y0=ws[2]))
+
+
+def test_linear_solve_loop_matches_dense_chained_solves(gpu):
+    """linear_solve_loop (solver.py:908-919): max_iter chained solves x <- K⁻¹ x."""
+    prob = mass_chain_mpc(horizon=10)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=3, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
+    solver = compile_osqp(plan, np.float32)
+    b = 5
+    rng = np.random.default_rng(9)
+    pv = plan.p_upper.data[None] * rng.uniform(0.5, 2.0, size=(b, plan.nnz_p))
+    av = np.broadcast_to(plan.a_matrix.data, (b, plan.nnz_a))
+    lx, dinv = solver.factor(pv, av)
+    rhs = rng.normal(size=(b, plan.qdldl_plan.n))
+    got = solver.linear_solve_loop(lx, dinv, rhs).cpu().numpy()
+    n, m = plan.n, plan.m
+    for i in range(b):
+        pu = sp.csc_matrix((pv[i], plan.p_upper.indices, plan.p_upper.indptr), shape=(n, n)).toarray()
+        pf = np.triu(pu) + np.triu(pu, 1).T
+        a = plan.a_matrix.toarray()
+        kkt = np.block([[pf + s.sigma * np.eye(n), a.T], [a, -np.diag(plan.rho_inv_vec)]])
+        x = rhs[i].astype(np.float32).astype(np.float64)
+        for _ in range(s.max_iter):
+            x = np.linalg.solve(kkt, x)
+        np.testing.assert_allclose(got[i], x, rtol=0, atol=1e-3 * np.abs(x).max())
+
+
+def test_work_size_is_exactly_the_documented_one(gpu, chain):
+    """wmpc_osqp_solve_f32 with work of exactly wmpc_osqp_work_floats floats
+    runs; one float less is rejected before any launch."""
+    import torch
+
+    from warpmpc_amd import _lib
+
+    prob, plan, solver = chain
+    lib = _lib.load()
+    dev = solver.extras["device_plan"]
+    b = 70
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=3)
+    f32 = dict(dtype=torch.float32, device="cuda")
+    sym = lambda v: torch.as_tensor(np.ascontiguousarray(np.asarray(v).T), **f32)  # noqa: E731
+    t = [sym(v) for v in (pv, av, q, l, u)]
+    outs = [torch.empty((plan.n, b), **f32), torch.empty((plan.m, b), **f32), torch.empty((plan.m, b), **f32)] + \
+        [torch.empty((b,), **f32) for _ in range(3)]
+    need = int(lib.wmpc_osqp_work_floats(dev.handle, 1, b))
+    assert need == 2 * plan.qdldl_plan.n * b + plan.qdldl_plan.nnz_l + plan.qdldl_plan.n
+    work = torch.empty((need,), **f32)
+    args = lambda wl: (dev.handle, *(x.data_ptr() for x in t[:2]), 0, *(x.data_ptr() for x in t[2:]),  # noqa: E731
+                       None, None, None, None, None, 0, work.data_ptr(), wl, *(o.data_ptr() for o in outs),
+                       25, b, b, torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.wmpc_osqp_solve_f32(*args(need)), "solve")
+    ref = solver.solve(pv, av, q, l, u)
+    np.testing.assert_array_equal(outs[0].t().cpu().numpy(), ref[0].cpu().numpy())
+    assert lib.wmpc_osqp_solve_f32(*args(need - 1)) != 0
+    assert "wmpc_osqp_work_floats" in lib.wmpc_last_error().decode()

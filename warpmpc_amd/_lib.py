@@ -203,10 +203,10 @@ def check(rc: int, what: str) -> None:
 
 def plan_slots(handle) -> dict[str, int]:
     """LDS slot counts of a QDLDL plan (``wmpc_qdldl_plan_slots``)."""
-    out = (ctypes.c_int64 * 9)()
-    check(load().wmpc_qdldl_plan_slots(handle, out, 9), "wmpc_qdldl_plan_slots")
+    out = (ctypes.c_int64 * 10)()
+    check(load().wmpc_qdldl_plan_slots(handle, out, 10), "wmpc_qdldl_plan_slots")
     keys = ("reach", "colwin", "live_l", "backwin", "live_l_needed", "global_rereads", "lds_factor_solve",
-            "lds_solve", "lds_budget")
+            "lds_solve", "lds_budget", "lds_columns")
     return dict(zip(keys, (int(v) for v in out)))
 
 

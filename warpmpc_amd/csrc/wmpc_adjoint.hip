@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
   // past the batch on the last instance
   int64_t b0;
   const int64_t b = lane_instance(ipw, batch, &b0);
-  float* mylds = lds + lane;
+  float* mywin = lds + (lane & (P.lcols - 1));
   const int64_t pes = pm_ld ? pm_ld : 1;
   const uint32_t pl = pm_ld ? (uint32_t)b : 0u;
   const CStrided X{x, ld, b}, Y{y, ld, b}, Lo{l, ld, b}, Up{u, ld, b};
@@ -155,10 +155,10 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
   const AdjRhs rhs{dv.rsrc, CStrided{dx, ld, b}, CStrided{dy, ld, b}, Y};
   // 2. factor M with the forward substitution of rhs fused in
   factor_rows<true>(P, msrc, 0, Strided{lx, ld, b}, Strided{nullptr, 0, 0}, Strided{dinv, ld, b},
-                    mylds, rhs, Strided{stash, ld, b});
+                    mywin, rhs, Strided{stash, ld, b});
   // 3. backward sweep, then the refinement against M0 (solver.py:1127-1139)
   const SweepProg SP = sweep_prog(P);
-  bwd_cols<false>(SP, lx, ld, b, stash, ld, b, mylds, lane, SolSet{SOL});
+  bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSet{SOL});
   for (int32_t it = 0; it < refine; ++it) {
     // residual M0 sol - rhs, staged in the sweep's first-touch order
     for (int32_t i = 0; i < dv.dim; ++i) {
@@ -170,8 +170,8 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
       }
       RB(i) = s - rhs.at(o);
     }
-    fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, mylds, lane);
-    bwd_cols<false>(SP, lx, ld, b, stash, ld, b, mylds, lane, SolSub{SOL});
+    fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, lds, lane);
+    bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSub{SOL});
   }
   // 4. outputs (solver.py:1141-1192); only real lanes store
   if (b0 >= batch) return;
@@ -210,8 +210,9 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
   }
 }
 
-static int32_t adjoint_lane_slots(const QdldlProg& P) {
-  return std::max(factor_lds_slots(P, true), sweep_lane_slots(sweep_prog(P), false));
+static size_t adjoint_lds(const QdldlProg& P) {
+  return (size_t)std::max(factor_lds_floats(P, true), sweep_lds_floats(sweep_prog(P), false)) *
+         sizeof(float);
 }
 
 }  // namespace wmpc
@@ -441,7 +442,7 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
   float* solw = rbuf + dim * ld;
   float* lam = solw + dim * ld;
   float* slk = lam + (int64_t)dv.ni * ld;
-  const size_t lds = (size_t)adjoint_lane_slots(P) * kWave * sizeof(float) + sweep_wave_bytes();
+  const size_t lds = adjoint_lds(P);
   if (lds > kMaxLdsBytes) {
     set_error("adjoint pattern needs more LDS than a workgroup can have");
     return WMPC_ERR_UNSUPPORTED;
@@ -449,7 +450,7 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
   if (lds > 64 * 1024)
     WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, P);
   hipLaunchKernelGGL(k_osqp_adjoint, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, P, dv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx,
                      dinv, stash, rbuf, sol ? sol : solw, lam, slk, dq, dl, du, dp, da, refine, batch,

@@ -50,6 +50,9 @@ inline int check_launch(const char* what) {
 // only add scalar work per instance; more CUs add scalar units.
 constexpr int64_t kFillWaves = 256;
 int32_t instances_per_wave(int64_t batch);
+struct QdldlProg;
+// ... capped by the LDS columns of a pattern's programs
+int32_t instances_per_wave(int64_t batch, const QdldlProg& P);
 inline int64_t wave_grid(int64_t batch, int32_t ipw) { return (batch + ipw - 1) / ipw; }
 
 // Largest dynamic LDS a workgroup may request on gfx950 (160 KiB per CU).

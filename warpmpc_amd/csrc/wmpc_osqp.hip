@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
   // past the batch on the last instance (identical loads and stores)
   int64_t b0;
   const int64_t b = lane_instance(ipw, batch, &b0);
-  float* mylds = lds + lane;
+  float* mywin = lds + (lane & (P.lcols - 1));
   const int64_t pb = pm_ld ? b : 0;
   const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
   const int32_t n = dv.n, m = dv.m;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
   if constexpr (FACTOR) {
     KktSource src{pv, av, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
     factor_rows<false>(P, src, pb, Strided{lx, fs, fb}, Strided{nullptr, 0, 0},
-                       Strided{dinv, fs, fb}, mylds, NoRhs{}, Strided{nullptr, 0, 0});
+                       Strided{dinv, fs, fb}, mywin, NoRhs{}, Strided{nullptr, 0, 0});
   }
   const Strided X{x, ld, b}, Z{z, ld, b}, Y{y, ld, b}, R{rbuf, ld, b};
   const CStrided Q{q, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, S{w, ld, b};
@@ -151,8 +151,8 @@ __global__ __launch_bounds__(kWave) void k_osqp_admm(
     const SweepProg SP = sweep_prog(P);
     admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
     for (int32_t it = 0; it < max_iter; ++it) {
-      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, mylds, lane);
-      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, mylds, lane, StashOut{w, ld, b});
+      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, lds, lane);
+      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, lds, lane, StashOut{w, ld, b});
       if (it + 1 < max_iter)
         admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
       else
@@ -208,19 +208,17 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
   if (b0 >= batch) return;
   const int64_t pb = src.pm_ld ? b : 0;
   factor_rows<false>(P, src, pb, Strided{lx, ld, b}, Strided{nullptr, 0, 0},
-                     Strided{dinv, ld, b}, lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
+                     Strided{dinv, ld, b}, lds + (threadIdx.x & (P.lcols - 1)), NoRhs{},
+                     Strided{nullptr, 0, 0});
 }
 
 static size_t factor_lds(const QdldlProg& P) {
-  return (size_t)factor_lds_slots(P, false) * kWave * sizeof(float);
-}
-static int32_t admm_lane_slots(const QdldlProg& P, bool shared, bool factor) {
-  int32_t s = sweep_lane_slots(sweep_prog(P), shared);
-  if (factor) s = std::max(s, factor_lds_slots(P, false));
-  return s;
+  return (size_t)factor_lds_floats(P, false) * sizeof(float);
 }
 static size_t admm_lds(const QdldlProg& P, bool shared, bool factor) {
-  return (size_t)admm_lane_slots(P, shared, factor) * kWave * sizeof(float) + sweep_wave_bytes();
+  int64_t f = sweep_lds_floats(sweep_prog(P), shared);
+  if (factor) f = std::max(f, factor_lds_floats(P, false));
+  return (size_t)f * sizeof(float);
 }
 
 template <class K>
@@ -399,7 +397,7 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
   const size_t lds = factor_lds(P);
   int rc = osqp_prepare_lds(k_osqp_factor, lds);
   if (rc) return rc;
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, P);
   KktSource src{p_values, a_values, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
   hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, P, src, lx, dinv, batch, ld, ipw);
@@ -431,7 +429,7 @@ static int osqp_solve(const wmpc_osqp_plan* plan, const float* p_values, const f
   const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
   const int64_t nk = P.n, nl = P.nnz_l;
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, P);
   const int64_t grid = wave_grid(batch, ipw);
   hipStream_t st = (hipStream_t)stream;
   float* stash = work;

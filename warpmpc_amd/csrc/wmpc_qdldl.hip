@@ -28,6 +28,10 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
 static std::atomic<int32_t> g_ipw{0};
+int32_t instances_per_wave(int64_t batch, const QdldlProg& P) {
+  return std::min(instances_per_wave(batch), P.lcols);
+}
+
 int32_t instances_per_wave(int64_t batch) {
   const int32_t forced = g_ipw.load(std::memory_order_relaxed);
   if (forced) return forced;
@@ -49,7 +53,7 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor(QdldlProg P, DirectSourc
   const int64_t b = lane_instance(ipw, batch, &b0);
   if (b0 >= batch) return;
   factor_rows<false>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
-                     lds + threadIdx.x, NoRhs{}, Strided{nullptr, 0, 0});
+                     lds + (threadIdx.x & (P.lcols - 1)), NoRhs{}, Strided{nullptr, 0, 0});
 }
 
 __global__ __launch_bounds__(kWave) void k_qdldl_factor_solve(
@@ -64,9 +68,9 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor_solve(
   const int64_t b = lane_instance(ipw, batch, &b0);
   const Strided stash{w, ld, b};
   factor_rows<true>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
-                    lds + lane, ArrayRhs{CStrided{rhs, ld, b}}, stash);
+                    lds + (lane & (P.lcols - 1)), ArrayRhs{CStrided{rhs, ld, b}}, stash);
   const SweepProg S = sweep_prog(P);
-  bwd_cols<false>(S, lx, ld, b, w, ld, b, lds + lane, lane, ArrayOut{Strided{x, ld, b}});
+  bwd_cols<false>(S, lx, ld, b, w, ld, b, lds, lane, ArrayOut{Strided{x, ld, b}});
 }
 
 // SHARED: one factor ([nnz_l], [n]) for the whole batch.
@@ -86,8 +90,8 @@ __global__ __launch_bounds__(kWave) void k_qdldl_solve(QdldlProg P, const float*
 #pragma unroll 8
   for (int32_t i = 0; i < P.n; ++i) x[(int64_t)i * ld + b] = rhs[(int64_t)P.rperm[i] * ld + b];
   const SweepProg S = sweep_prog(P);
-  fwd_cols<SHARED>(S, lx, dinv, fs, fb, x, w, ld, b, lds + lane, lane);
-  bwd_cols<SHARED>(S, lx, fs, fb, w, ld, b, lds + lane, lane, ArrayOut{Strided{x, ld, b}});
+  fwd_cols<SHARED>(S, lx, dinv, fs, fb, x, w, ld, b, lds, lane);
+  bwd_cols<SHARED>(S, lx, fs, fb, w, ld, b, lds, lane, ArrayOut{Strided{x, ld, b}});
 }
 
 // ---------------------------------------------------------------------------
@@ -158,20 +162,31 @@ static int upload_program(wmpc_qdldl_plan* plan) {
   return WMPC_OK;
 }
 
-// Derive the programs for the plan's LDS budget: live-L slots get whatever the
-// windows of the fused factor+solve leave.
-// A default-budget plan whose windows alone exceed the budget gets the smallest
-// budget (in 4 KiB steps) that holds them; an explicit budget must fit.
+// Derive the programs for the plan's LDS budget (bytes of factor windows per
+// workgroup): live-L slots get whatever the windows of the fused factor+solve
+// leave.  The windows take `lcols` LDS columns: 64 (one per lane) when the
+// factor windows and the sweeps fit in the largest workgroup LDS, else the
+// largest of 32, 16, 8 that fits, the lanes of one instance sharing a column.
+// A default-budget plan whose factor windows exceed the budget gets the
+// smallest budget (in 4 KiB steps) that holds them; an explicit budget must.
 static int derive(wmpc_qdldl_plan* plan, bool upload = true, bool explicit_budget = false) {
   HostProgram probe;
   std::string err = build_program(&plan->pattern, 0, probe);
   WMPC_REQUIRE(err.empty(), err.c_str());
-  const int64_t fixed = probe.ymax + 2 * (int64_t)probe.ncol + sweep_wave_bytes() / (kWave * 4) + 1;
+  const int64_t fixed = probe.ymax + 2 * (int64_t)probe.ncol + 1;  // factor windows, in slots
+  const int64_t ring = (int64_t)(kLPer + 2 * kVPer) * kWave;         // sweep chunk rings, floats
+  auto need_floats = [&](int64_t lc) { return std::max(fixed * lc, probe.nbw * lc + ring); };
+  const int64_t limit = (int64_t)kMaxLdsBytes / 4;
+  int32_t lc = kWave;
+  while (lc > 8 && need_floats(lc) > limit) lc /= 2;
+  WMPC_REQUIRE(need_floats(lc) <= limit,
+               "the row windows of this pattern exceed a workgroup's LDS even at 8 instances "
+               "per workgroup");
   if (!explicit_budget) {
-    const size_t need = ((size_t)fixed * kWave * sizeof(float) + 4095) / 4096 * 4096;
+    const size_t need = ((size_t)(fixed * lc) * sizeof(float) + 4095) / 4096 * 4096;
     if (need > plan->lds_budget) plan->lds_budget = std::min(need, kMaxLdsBytes);
   }
-  const int64_t slots = (int64_t)(plan->lds_budget / (kWave * sizeof(float)));
+  const int64_t slots = (int64_t)(plan->lds_budget / (lc * sizeof(float)));
   WMPC_REQUIRE(slots >= fixed, "LDS budget too small for the row windows of this pattern");
   int32_t cap = (int32_t)std::min<int64_t>(slots - fixed, INT32_MAX);
   err = build_program(&plan->pattern, cap, plan->host);
@@ -184,25 +199,22 @@ static int derive(wmpc_qdldl_plan* plan, bool upload = true, bool explicit_budge
     plan->prog.nact = plan->host.nact;
     plan->prog.nbw = plan->host.nbw;
     plan->prog.nnz_l = plan->host.nnz_l;
+    plan->prog.lcols = lc;
     return WMPC_OK;
   }
-  return upload_program(plan);
+  const int rc = upload_program(plan);
+  plan->prog.lcols = lc;
+  return rc;
 }
 
-// per-lane LDS slots and total bytes of the kernels
-static int32_t lane_slots_factor(const QdldlProg& P, bool fwd) {
-  int32_t s = factor_lds_slots(P, fwd);
-  if (fwd) s = std::max(s, sweep_lane_slots(sweep_prog(P), false));
-  return s;
-}
+// LDS bytes of the kernels
 static size_t lds_bytes_factor(const QdldlProg& P, bool fwd) {
-  return (size_t)lane_slots_factor(P, fwd) * kWave * sizeof(float) + (fwd ? sweep_wave_bytes() : 0);
-}
-static int32_t lane_slots_solve(const QdldlProg& P, bool shared) {
-  return sweep_lane_slots(sweep_prog(P), shared);
+  int64_t f = factor_lds_floats(P, fwd);
+  if (fwd) f = std::max(f, sweep_lds_floats(sweep_prog(P), false));
+  return (size_t)f * sizeof(float);
 }
 static size_t lds_bytes_solve(const QdldlProg& P, bool shared) {
-  return (size_t)lane_slots_solve(P, shared) * kWave * sizeof(float) + sweep_wave_bytes();
+  return (size_t)sweep_lds_floats(sweep_prog(P), shared) * sizeof(float);
 }
 
 template <class K>
@@ -326,6 +338,7 @@ int wmpc_qdldl_plan_slots(const wmpc_qdldl_plan* plan, int64_t* out, int32_t len
   out[6] = (int64_t)lds_bytes_factor(plan->prog, true);
   out[7] = (int64_t)lds_bytes_solve(plan->prog, false);
   out[8] = (int64_t)plan->lds_budget;
+  if (len >= 10) out[9] = plan->prog.lcols;
   return WMPC_OK;
 }
 
@@ -356,7 +369,7 @@ int wmpc_qdldl_factor_f32(const wmpc_qdldl_plan* plan, const float* a_values, fl
   const size_t lds = lds_bytes_factor(plan->prog, false);
   int rc = prepare_lds(k_qdldl_factor, lds);
   if (rc) return rc;
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, plan->prog);
   hipLaunchKernelGGL(k_qdldl_factor, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, plan->prog, DirectSource{a_values, ld}, lx, d, dinv, batch,
                      ld, ipw);
@@ -377,7 +390,7 @@ int wmpc_qdldl_solve_f32(const wmpc_qdldl_plan* plan, const float* lx, int64_t l
   WMPC_REQUIRE(work_len >= (int64_t)plan->prog.n * ld, "work must hold n * ld floats");
   const bool shared = lx_ld == 0;
   const size_t lds = lds_bytes_solve(plan->prog, shared);
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, plan->prog);
   const int64_t grid = wave_grid(batch, ipw);
   WMPC_REQUIRE(x != rhs, "x must not alias rhs (x stages the right-hand side)");
   if (shared) {
@@ -410,7 +423,7 @@ int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_valu
   const size_t lds = lds_bytes_factor(plan->prog, true);
   int rc = prepare_lds(k_qdldl_factor_solve, lds);
   if (rc) return rc;
-  const int32_t ipw = instances_per_wave(batch);
+  const int32_t ipw = instances_per_wave(batch, plan->prog);
   hipLaunchKernelGGL(k_qdldl_factor_solve, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, plan->prog, DirectSource{a_values, ld}, rhs, x, lx, d,
                      dinv, work, batch, ld, ipw);

@@ -52,13 +52,20 @@ struct QdldlProg {
   const int32_t* brec = nullptr;
   int32_t fchunks = 0, bchunks = 0;
   int32_t f1[3] = {0, 0, 0}, b1[3] = {0, 0, 0};
+  // LDS columns of the slot windows: 64 (one per lane), or 32/16/8 for patterns
+  // whose windows do not fit otherwise; lane l uses column l mod lcols, so a
+  // launch serves at most lcols instances per wavefront
+  int32_t lcols = kWave;
 };
 
-// LDS floats per lane of the factorization (fused forward adds one window).
+// LDS slots of the factorization (fused forward adds one window) and the
+// floats they take in a workgroup.
 __host__ __device__ inline int32_t factor_lds_slots(const QdldlProg& P, bool fwd) {
   return P.ymax + P.ncol * (fwd ? 2 : 1) + P.nact;
 }
-__host__ __device__ inline int32_t solve_lds_slots(const QdldlProg& P) { return P.ncol + P.nbw; }
+__host__ __device__ inline int64_t factor_lds_floats(const QdldlProg& P, bool fwd) {
+  return (int64_t)factor_lds_slots(P, fwd) * P.lcols;
+}
 
 // Values straight from a [nnz_a][ld] array (CompiledQDLDL.factor input).
 struct DirectSource {
@@ -125,99 +132,61 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
                                             const Strided& lx, const Strided& d,
                                             const Strided& dinv, float* lds, const Rhs& rhs,
                                             const Strided& stash) {
+  const int32_t lc = P.lcols;
   float* Y = lds;
-  float* CW = Y + P.ymax * kWave;
-  float* AL = CW + P.ncol * kWave;
-  float* FW = AL + P.nact * kWave;
+  float* CW = Y + P.ymax * lc;
+  float* AL = CW + P.ncol * lc;
+  float* FW = AL + P.nact * lc;
   int32_t r = 0, j = 0, u = 0;
   for (int32_t k = 0; k < P.n; ++k) {
     const int4 ri = P.row[k];
     const int32_t r1 = ri.x;
-    for (int32_t s = 0; s < r1 - r; ++s) Y[s * kWave] = 0.f;
+    for (int32_t s = 0; s < r1 - r; ++s) Y[s * lc] = 0.f;
     for (; j < ri.y; ++j) {
       const int2 e = P.init[j];
-      Y[e.x * kWave] += src(e.y, sb);
+      Y[e.x * lc] += src(e.y, sb);
     }
     float dk = src(ri.w, sb);
     float fw = 0.f;
     if constexpr (FWD) fw = rhs(k, P.p[k]);
     for (int32_t s = 0; r < r1; ++r, ++s) {
       const int4 pe = P.proc[r];
-      const float val = Y[s * kWave];
+      const float val = Y[s * lc];
       // the updates of one reach entry hit distinct later slots: issue their
       // loads in groups so several LDS / global round trips overlap
       for (; u + 4 <= pe.w; u += 4) {
         const int2 e0 = P.upd[u], e1 = P.upd[u + 1], e2 = P.upd[u + 2], e3 = P.upd[u + 3];
-        const float l0 = e0.y >= 0 ? AL[e0.y * kWave] : lx(-(int64_t)e0.y - 1);
-        const float l1 = e1.y >= 0 ? AL[e1.y * kWave] : lx(-(int64_t)e1.y - 1);
-        const float l2 = e2.y >= 0 ? AL[e2.y * kWave] : lx(-(int64_t)e2.y - 1);
-        const float l3 = e3.y >= 0 ? AL[e3.y * kWave] : lx(-(int64_t)e3.y - 1);
-        const float y0 = Y[e0.x * kWave], y1 = Y[e1.x * kWave];
-        const float y2 = Y[e2.x * kWave], y3 = Y[e3.x * kWave];
-        Y[e0.x * kWave] = y0 - l0 * val;
-        Y[e1.x * kWave] = y1 - l1 * val;
-        Y[e2.x * kWave] = y2 - l2 * val;
-        Y[e3.x * kWave] = y3 - l3 * val;
+        const float l0 = e0.y >= 0 ? AL[e0.y * lc] : lx(-(int64_t)e0.y - 1);
+        const float l1 = e1.y >= 0 ? AL[e1.y * lc] : lx(-(int64_t)e1.y - 1);
+        const float l2 = e2.y >= 0 ? AL[e2.y * lc] : lx(-(int64_t)e2.y - 1);
+        const float l3 = e3.y >= 0 ? AL[e3.y * lc] : lx(-(int64_t)e3.y - 1);
+        const float y0 = Y[e0.x * lc], y1 = Y[e1.x * lc];
+        const float y2 = Y[e2.x * lc], y3 = Y[e3.x * lc];
+        Y[e0.x * lc] = y0 - l0 * val;
+        Y[e1.x * lc] = y1 - l1 * val;
+        Y[e2.x * lc] = y2 - l2 * val;
+        Y[e3.x * lc] = y3 - l3 * val;
       }
       for (; u < pe.w; ++u) {
         const int2 e = P.upd[u];
-        const float l = e.y >= 0 ? AL[e.y * kWave] : lx(-(int64_t)e.y - 1);
-        Y[e.x * kWave] -= l * val;
+        const float l = e.y >= 0 ? AL[e.y * lc] : lx(-(int64_t)e.y - 1);
+        Y[e.x * lc] -= l * val;
       }
-      const float lval = val * CW[pe.x * kWave];
+      const float lval = val * CW[pe.x * lc];
       lx(pe.y) = lval;
-      if (pe.z >= 0) AL[pe.z * kWave] = lval;
+      if (pe.z >= 0) AL[pe.z * lc] = lval;
       dk -= val * lval;
-      if constexpr (FWD) fw -= lval * FW[pe.x * kWave];
+      if constexpr (FWD) fw -= lval * FW[pe.x * lc];
     }
     if (d.ptr) d(k) = dk;
     const float di = 1.0f / dk;
     dinv(k) = di;
     if (ri.z >= 0) {
-      CW[ri.z * kWave] = di;
-      if constexpr (FWD) FW[ri.z * kWave] = fw;
+      CW[ri.z * lc] = di;
+      if constexpr (FWD) FW[ri.z * lc] = fw;
     }
     // the backward sweep consumes the D⁻¹-scaled forward result
     if constexpr (FWD) stash(k) = fw * di;
-  }
-}
-
-// Forward substitution by rows: w[k] = rhs(k, p[k]) - sum_c L[k,c] w[c], stored
-// to stash(k); FW is the column window (slots of the factor program).
-// (Reference form; the kernels use the prefetched column sweeps of
-// wmpc_stream.h.)
-template <class LAcc, class Rhs>
-__device__ __forceinline__ void forward_rows(const QdldlProg& P, const LAcc& L, const Rhs& rhs,
-                                             const Strided& stash, float* FW) {
-  int32_t r = 0;
-  for (int32_t k = 0; k < P.n; ++k) {
-    const int4 ri = P.row[k];
-    float acc = rhs(k, P.p[k]);
-    for (; r < ri.x; ++r) {
-      const int4 a = P.proc[r];
-      acc -= L(a.y) * FW[a.x * kWave];
-    }
-    if (ri.z >= 0) FW[ri.z * kWave] = acc;
-    stash(k) = acc;
-  }
-}
-
-// D⁻¹ scaling fused with the backward substitution by columns, c = n-1 .. 0:
-// w[c] = stash(c) * dinv(c) - sum_q L[q] w[li[q]]; out(c, p[c], w[c]) receives
-// the solution entry of original index p[c].
-template <class LAcc, class DAcc, class Out>
-__device__ __forceinline__ void backward_cols(const QdldlProg& P, const LAcc& L, const DAcc& dinv,
-                                              const Strided& stash, float* BW, const Out& out) {
-  for (int32_t c = P.n - 1; c >= 0; --c) {
-    const int2 bc = P.bcol[c];
-    int32_t q = c > 0 ? P.bcol[c - 1].y : 0;
-    // same order as core.py:1003-1028: the correction is summed, then
-    // subtracted from the D⁻¹-scaled forward value
-    float corr = 0.f;
-    for (; q < bc.y; ++q) corr += L(q) * BW[P.bent[q] * kWave];
-    const float acc = stash(c) * dinv(c) - corr;
-    if (bc.x >= 0) BW[bc.x * kWave] = acc;
-    out(c, P.p[c], acc);
   }
 }
 

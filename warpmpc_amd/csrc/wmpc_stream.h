@@ -53,6 +53,7 @@ struct RecStream {
 };
 struct SweepProg {
   int32_t n = 0, nnz_l = 0, nw = 0;
+  int32_t lcols = 64;  // LDS columns of the row window (QdldlProg::lcols)
   RecStream fwd, bwd;
 };
 
@@ -69,18 +70,19 @@ __host__ __device__ inline SweepProg sweep_prog(const QdldlProg& P) {
   S.n = P.n;
   S.nnz_l = P.nnz_l;
   S.nw = P.nbw;
+  S.lcols = P.lcols;
   S.fwd = RecStream{reinterpret_cast<const I2*>(P.frec), P.fchunks, P.f1[0], P.f1[1], P.f1[2]};
   S.bwd = RecStream{reinterpret_cast<const I2*>(P.brec), P.bchunks, P.b1[0], P.b1[1], P.b1[2]};
   return S;
 }
 
-// LDS of the sweeps: per-lane floats (W window, then the current chunk's L
-// values of a per-instance factor and its per-column values); wave-uniform
+// LDS floats of the sweeps in a workgroup: the W window ([slot][lcols]), then
+// the current chunk's L values of a per-instance factor and its per-column
+// values, one column per lane ([slot][64], constant offsets); wave-uniform
 // chunk data stays in registers.
-__host__ __device__ inline int32_t sweep_lane_slots(const SweepProg& S, bool shared_l) {
-  return S.nw + (shared_l ? kVPer : kLPer + 2 * kVPer);
+__host__ __device__ inline int64_t sweep_lds_floats(const SweepProg& S, bool shared_l) {
+  return (int64_t)S.nw * S.lcols + (int64_t)(shared_l ? kVPer : kLPer + 2 * kVPer) * kWave;
 }
-__host__ __device__ inline int32_t sweep_wave_bytes() { return 0; }
 
 // One chunk of a sweep in flight: loaded into registers two chunks ahead of
 // use.  Every chunk issues the same loads, so the compiler's wait counts
@@ -108,18 +110,20 @@ __device__ __forceinline__ float rlf(float v, int32_t k) {
 
 template <bool SHL, int NV>
 struct Sweep {
-  float* W;      // lane column: row window
+  float* W;      // lane column: row window, slot s at W[s * lc]
   float* lring;  // lane column: L values of the current chunk (per-instance factor)
   float* vring;  // lane column: NV value streams of the current chunk
   int lane;
+  int32_t lc;
   // the current chunk's wave-uniform data
   int32_t crx, cry;
   float clu, cdu;
 
-  __device__ __forceinline__ void setup(float* lane_lds, int32_t nw, int lane_) {
+  __device__ __forceinline__ void setup(float* lds_base, int32_t nw, int32_t lc_, int lane_) {
     lane = lane_;
-    W = lane_lds;
-    lring = W + nw * kWave;
+    lc = lc_;
+    W = lds_base + (lane_ & (lc_ - 1));
+    lring = lds_base + (int64_t)nw * lc_ + lane_;
     vring = lring + (SHL ? 0 : kLPer * kWave);
   }
 
@@ -202,7 +206,7 @@ __device__ __forceinline__ int32_t fwd_group(Sweep<SHL, NV>& sw, int32_t k, int3
     l[g] = sw.L(j + g);
   }
 #pragma unroll
-  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * kWave];
+  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * sw.lc];
   int32_t used = 0;
   if (fa) {
 #pragma unroll
@@ -214,7 +218,7 @@ __device__ __forceinline__ int32_t fwd_group(Sweep<SHL, NV>& sw, int32_t k, int3
     }
   }
 #pragma unroll
-  for (int g = 0; g < G; ++g) sw.W[sl[g] * kWave] = w[g] - l[g] * val;
+  for (int g = 0; g < G; ++g) sw.W[sl[g] * sw.lc] = w[g] - l[g] * val;
   return used;
 }
 
@@ -228,7 +232,7 @@ __device__ __forceinline__ float bwd_group(const Sweep<SHL, NV>& sw, int32_t k, 
     l[g] = sw.L(j + g);
   }
 #pragma unroll
-  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * kWave];
+  for (int g = 0; g < G; ++g) w[g] = sw.W[sl[g] * sw.lc];
   float acc = 0.f;
 #pragma unroll
   for (int g = 0; g < G; ++g) acc += l[g] * w[g];
@@ -267,10 +271,10 @@ template <bool SHL>
 __device__ __forceinline__ void fwd_cols(const SweepProg& S, const float* __restrict__ lx,
                                          const float* __restrict__ dinv, int64_t f_ld, int64_t fb,
                                          const float* __restrict__ rhs, float* __restrict__ stash,
-                                         int64_t ld, int64_t b, float* lane_lds, int lane) {
+                                         int64_t ld, int64_t b, float* lds_base, int lane) {
   constexpr int NV = SHL ? 1 : 2;
   Sweep<SHL, NV> sw;
-  sw.setup(lane_lds, S.nw, lane);
+  sw.setup(lds_base, S.nw, S.lcols, lane);
   const RecStream R = S.fwd;
   const gptr<const I2> rec = opaque(R.rec);
   const gptr<float> st = (gptr<float>)opaque(stash);
@@ -295,7 +299,7 @@ __device__ __forceinline__ void fwd_cols(const SweepProg& S, const float* __rest
       if (rw & kMarkBit) {
         const int32_t slot = rw & kSlotMask;
         if (slot != kNoSlot) {
-          val = sw.W[slot * kWave];
+          val = sw.W[slot * sw.lc];
         } else {
           val = sw.V(0, vi - vb);
           dv = 1;
@@ -349,10 +353,10 @@ __device__ __forceinline__ void fwd_cols(const SweepProg& S, const float* __rest
 template <bool SHL, class Out>
 __device__ __forceinline__ void bwd_cols(const SweepProg& S, const float* __restrict__ lx,
                                          int64_t f_ld, int64_t fb, const float* __restrict__ stash,
-                                         int64_t ld, int64_t b, float* lane_lds, int lane,
+                                         int64_t ld, int64_t b, float* lds_base, int lane,
                                          const Out& out) {
   Sweep<SHL, 1> sw;
-  sw.setup(lane_lds, S.nw, lane);
+  sw.setup(lds_base, S.nw, S.lcols, lane);
   const RecStream R = S.bwd;
   const gptr<const I2> rec = opaque(R.rec);
   const int32_t nl = S.nnz_l > 0 ? S.nnz_l : 1;
@@ -375,7 +379,7 @@ __device__ __forceinline__ void bwd_cols(const SweepProg& S, const float* __rest
         dv = 1;
         corr = 0.f;
         const int32_t slot = rw & kSlotMask;
-        if (slot != kNoSlot) sw.W[slot * kWave] = w;
+        if (slot != kNoSlot) sw.W[slot * sw.lc] = w;
         out(col, sw.rec_a(k), w);
         --col;
       } else {
