@@ -202,6 +202,53 @@ int wmpc_osqp_solve_f32(const wmpc_osqp_plan* plan, const float* p_values,
                         float* obj, int32_t max_iter, int64_t batch, int64_t ld, void* stream);
 
 /*
+ * Heterogeneous batch: several fixed-pattern OSQP problem sets (different
+ * plans, each with its own batch) solved in one persistent launch.  No
+ * counterpart in the reference, which solves one pattern per compiled solver;
+ * each instance's arithmetic is that of wmpc_osqp_solve_f32 with a per-instance
+ * factor (solver.py:822-872), so results equal one wmpc_osqp_solve_f32 call
+ * per pattern.  Patterns are scheduled largest-L first over a work queue.
+ */
+typedef struct wmpc_osqp_hetero_plan wmpc_osqp_hetero_plan;
+/* One pattern's arrays for one call, as in wmpc_osqp_solve_f32 (factor always
+ * computed per instance; pm_ld 0 = P/A shared by the pattern's batch). */
+typedef struct wmpc_osqp_hetero_io {
+  const float* p_values;
+  const float* a_values;
+  int64_t pm_ld;
+  const float* q;
+  const float* l;
+  const float* u;
+  const float* x0; /* warm start: x0, z0, y0 all given or all NULL */
+  const float* z0;
+  const float* y0;
+  float* x;
+  float* z;
+  float* y;
+  float* prim;
+  float* dual;
+  float* obj;
+  int64_t batch, ld;
+} wmpc_osqp_hetero_io;
+/* 1 to 12 OSQP plans (referenced, not copied; they must outlive it). */
+int wmpc_osqp_hetero_plan_create(const wmpc_osqp_plan* const* plans, int32_t count,
+                                 wmpc_osqp_hetero_plan** out);
+int wmpc_osqp_hetero_plan_destroy(wmpc_osqp_hetero_plan* plan);
+/* Floats of scratch for instance strides ld[0..count). -1 on bad input. */
+int64_t wmpc_osqp_hetero_work_floats(const wmpc_osqp_hetero_plan* plan, const int64_t* ld);
+/* io[k]: the arrays of plans[k]. */
+int wmpc_osqp_solve_hetero_f32(const wmpc_osqp_hetero_plan* plan, const wmpc_osqp_hetero_io* io,
+                               int32_t max_iter, float* work, int64_t work_len, void* stream);
+
+/* Row gather of batch-major arrays, out[i][b] = in[rows[i]][b] for i < nrows
+ * (rows: device int32 array; a row outside [0, in_rows) gives NaN): the
+ * one-stage shift of a previous MPC solution used as the next warm start.  No
+ * counterpart in the reference (its SQP warm start reuses the previous
+ * solution unshifted, sparse_mpc.py:1084-1085).  out must not alias in. */
+int wmpc_gather_rows_f32(const int32_t* rows, int32_t nrows, int32_t in_rows, const float* in,
+                         int64_t in_ld, float* out, int64_t out_ld, int64_t batch, void* stream);
+
+/*
  * wmpc_osqp_solve_f32 without a work argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The

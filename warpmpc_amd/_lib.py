@@ -130,6 +130,17 @@ class AdjointDesc(ctypes.Structure):
     ]
 
 
+class HeteroIO(ctypes.Structure):
+    """Mirror of ``wmpc_osqp_hetero_io``."""
+
+    _fields_ = [
+        ("p_values", _vp), ("a_values", _vp), ("pm_ld", ctypes.c_int64),
+        ("q", _vp), ("l", _vp), ("u", _vp), ("x0", _vp), ("z0", _vp), ("y0", _vp),
+        ("x", _vp), ("z", _vp), ("y", _vp), ("prim", _vp), ("dual", _vp), ("obj", _vp),
+        ("batch", ctypes.c_int64), ("ld", ctypes.c_int64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/warpmpc_hip.h declares
 SIGNATURES: dict[str, tuple] = {
     "wmpc_last_error": (ctypes.c_char_p, []),
@@ -154,6 +165,11 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_osqp_solve_f32": (ctypes.c_int, [
         _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
+    "wmpc_osqp_hetero_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _i32, ctypes.POINTER(_vp)]),
+    "wmpc_osqp_hetero_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_osqp_hetero_work_floats": (ctypes.c_int64, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "wmpc_osqp_solve_hetero_f32": (ctypes.c_int, [_vp, ctypes.POINTER(HeteroIO), _i32, _vp, _i64, _vp]),
+    "wmpc_gather_rows_f32": (ctypes.c_int, [_vp, _i32, _i32, _vp, _i64, _vp, _i64, _i64, _vp]),
     "wmpc_osqp_noalloc_max_floats": (ctypes.c_int64, []),
     "wmpc_osqp_solve_f32_noalloc": (ctypes.c_int, [
         _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
@@ -230,5 +246,5 @@ def f64p(a: np.ndarray):
     return a.ctypes.data_as(_f64p)
 
 
-__all__ = ["AdjointDesc", "HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
+__all__ = ["AdjointDesc", "HeteroIO", "HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
            "load", "plan_program", "plan_slots"]

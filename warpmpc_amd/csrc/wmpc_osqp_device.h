@@ -1,0 +1,192 @@
+// Device side of the fused OSQP solve (gfx950): the KKT value sources and
+// update-pass tables of one pattern, the ADMM update pass and one instance's
+// whole solve.  Shared by the single-pattern launch (wmpc_osqp.hip) and the
+// heterogeneous multi-pattern launch (wmpc_hetero.hip).
+#pragma once
+
+#include <algorithm>
+
+#include "wmpc_plan.h"
+#include "wmpc_stream.h"
+
+namespace wmpc {
+
+struct OsqpDev {
+  int32_t n = 0, m = 0, nnz_p = 0, nnz_a = 0, nnz_kkt = 0;
+  float sigma = 0.f, alpha = 0.f;
+  // KKT value sources, indexed by original KKT slot
+  const int32_t* src_kind = nullptr;
+  const int32_t* src_idx = nullptr;
+  const float* src_scale = nullptr;
+  const float* src_off = nullptr;
+  // update-pass entries: variables {c, o, first-touch position, q scale},
+  // constraints {c, i, first-touch position, rho⁻¹} + {rho, E, 0, 0}
+  int32_t nvar = 0, ncon = 0;
+  const int4* vtab = nullptr;
+  const int4* ctab = nullptr;
+  const float* sd = nullptr;      // D      [n]
+  const float* sdinv = nullptr;   // D⁻¹    [n]
+  const float* se = nullptr;      // E      [m]
+  const float* seinv = nullptr;   // E⁻¹    [m]
+  const float* y_scale = nullptr;    // [m]
+  const float* y_unscale = nullptr;  // [m]
+  // gather tables of the residuals: rows of A, columns of A, rows of full P
+  const int32_t* arow_ptr = nullptr;  // [m+1]
+  const int2* arow = nullptr;         // {col, A value slot}
+  const int32_t* acol_ptr = nullptr;  // [n+1]
+  const int2* acol = nullptr;         // {row, A value slot}
+  const int32_t* prow_ptr = nullptr;  // [n+1]
+  const int2* prow = nullptr;         // {col, P value slot}
+};
+
+}  // namespace wmpc
+
+struct wmpc_osqp_plan {
+  wmpc::OsqpDev d;
+  const wmpc_qdldl_plan* kkt = nullptr;
+  void* block = nullptr;
+};
+
+namespace wmpc {
+
+// Solution entry c of the backward sweep goes back into the stash.
+struct StashOut {
+  float* __restrict__ stash;
+  int64_t ld, b;
+  __device__ __forceinline__ void operator()(int32_t c, int32_t, float v) const {
+    (stash + (int64_t)c * ld)[(uint32_t)b] = v;
+  }
+};
+
+// One pass over the iterate (solver.py:806-820): unless FIRST, the relaxation,
+// projection and dual update with the KKT solution S; unless LAST, the next
+// right-hand side R (solver.py:808-810) at the entry's first-touch position.
+template <bool FIRST, bool LAST>
+__device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, const Strided& R,
+                                          const Strided& X, const Strided& Z, const Strided& Y,
+                                          const CStrided& Q, const CStrided& Lo,
+                                          const CStrided& Up) {
+  const float alpha = dv.alpha, oma = 1.0f - dv.alpha, sigma = dv.sigma;
+#pragma unroll 4
+  for (int32_t v = 0; v < dv.nvar; ++v) {
+    const int4 t = dv.vtab[v];
+    const float xo = X(t.y);
+    float xn = xo;
+    if constexpr (!FIRST) {
+      xn = alpha * S(t.x) + oma * xo;
+      X(t.y) = xn;
+    }
+    if constexpr (!LAST) R(t.z) = sigma * xn - __int_as_float(t.w) * Q(t.y);
+  }
+#pragma unroll 4
+  for (int32_t k = 0; k < dv.ncon; ++k) {
+    const int4 t = dv.ctab[2 * k];
+    const float rinv = __int_as_float(t.w);
+    const float zo = Z(t.y), yo = Y(t.y);
+    float zn = zo, yn = yo;
+    if constexpr (!FIRST) {
+      const int4 t2 = dv.ctab[2 * k + 1];
+      const float rho = __int_as_float(t2.x), e = __int_as_float(t2.y);
+      const float rz = zo - rinv * yo;
+      const float zt = rz + rinv * S(t.x);
+      const float zr = alpha * zt + oma * zo;
+      zn = nan_min(nan_max(zr + rinv * yo, Lo(t.y) * e), Up(t.y) * e);
+      yn = yo + rho * (zr - zn);
+      Z(t.y) = zn;
+      Y(t.y) = yn;
+    }
+    if constexpr (!LAST) R(t.z) = zn - rinv * yn;
+  }
+}
+
+// One solve of instance b (solver.py:822-872) by this lane; every lane of the
+// wavefront takes part (the sweeps stage operands wave-wide).
+// SHARED: one factor for the batch ([nnz_l], [n_kkt]); FACTOR: assemble and
+// factor each instance's KKT into the work area first.
+template <bool SHARED, bool FACTOR>
+__device__ __forceinline__ void admm_solve(
+    const QdldlProg& P, const OsqpDev& dv, const float* __restrict__ pv,
+    const float* __restrict__ av, int64_t pm_ld, const float* __restrict__ q,
+    const float* __restrict__ l, const float* __restrict__ u, const float* __restrict__ x0,
+    const float* __restrict__ z0, const float* __restrict__ y0, float* __restrict__ lx,
+    float* __restrict__ dinv, int64_t f_ld, float* __restrict__ w, float* __restrict__ rbuf,
+    float* __restrict__ x, float* __restrict__ z, float* __restrict__ y, float* __restrict__ prim,
+    float* __restrict__ dual, float* __restrict__ obj, int32_t max_iter, int64_t ld, int64_t b,
+    float* lds, int lane) {
+  float* mywin = lds + (lane & (P.lcols - 1));
+  const int64_t pb = pm_ld ? b : 0;
+  const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
+  const int32_t n = dv.n, m = dv.m;
+  const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
+  if constexpr (FACTOR) {
+    KktSource src{pv, av, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
+    factor_rows<false>(P, src, pb, Strided{lx, fs, fb}, Strided{nullptr, 0, 0},
+                       Strided{dinv, fs, fb}, mywin, NoRhs{}, Strided{nullptr, 0, 0});
+  }
+  const Strided X{x, ld, b}, Z{z, ld, b}, Y{y, ld, b}, R{rbuf, ld, b};
+  const CStrided Q{q, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, S{w, ld, b};
+  // scaled initial iterate (solver.py:842-847)
+  for (int32_t i = 0; i < n; ++i) X(i) = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
+  for (int32_t i = 0; i < m; ++i) {
+    Z(i) = z0 ? z0[(int64_t)i * ld + b] * dv.se[i] : 0.0f;
+    Y(i) = y0 ? y0[(int64_t)i * ld + b] * dv.y_scale[i] : 0.0f;
+  }
+  if (max_iter > 0) {
+    const SweepProg SP = sweep_prog(P);
+    admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+    for (int32_t it = 0; it < max_iter; ++it) {
+      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, lds, lane);
+      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, lds, lane, StashOut{w, ld, b});
+      if (it + 1 < max_iter)
+        admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+      else
+        admm_pass<false, true>(dv, S, R, X, Z, Y, Q, Lo, Up);
+    }
+  }
+  // unscale (solver.py:863-865)
+  for (int32_t i = 0; i < n; ++i) X(i) *= dv.sd[i];
+  for (int32_t i = 0; i < m; ++i) {
+    Z(i) *= dv.seinv[i];
+    Y(i) *= dv.y_unscale[i];
+  }
+  // residuals and objective with the unscaled P, A, q (solver.py:866-872), in
+  // gather form: no scatter scratch
+  float pr = 0.0f;
+  for (int32_t i = 0; i < m; ++i) {
+    float s = 0.0f;
+    for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
+      const int2 t = dv.arow[e];
+      s += av[(int64_t)t.y * pes + pb] * X(t.x);
+    }
+    pr = nan_max(pr, fabsf(s - Z(i)));
+  }
+  float du = 0.0f, quad = 0.0f, lin = 0.0f;
+  for (int32_t j = 0; j < n; ++j) {
+    float px = 0.0f, aty = 0.0f;
+    for (int32_t e = dv.prow_ptr[j]; e < dv.prow_ptr[j + 1]; ++e) {
+      const int2 t = dv.prow[e];
+      px += pv[(int64_t)t.y * pes + pb] * X(t.x);
+    }
+    for (int32_t e = dv.acol_ptr[j]; e < dv.acol_ptr[j + 1]; ++e) {
+      const int2 t = dv.acol[e];
+      aty += av[(int64_t)t.y * pes + pb] * Y(t.x);
+    }
+    const float xj = X(j), qj = Q(j);
+    du = nan_max(du, fabsf(px + qj + aty));
+    quad += xj * px;
+    lin += qj * xj;
+  }
+  prim[b] = pr;
+  dual[b] = du;
+  obj[b] = 0.5f * quad + lin;
+}
+
+// LDS bytes of one ADMM workgroup (sweeps, plus the factor windows when the
+// launch factors).
+inline size_t admm_lds(const QdldlProg& P, bool shared, bool factor) {
+  int64_t f = sweep_lds_floats(sweep_prog(P), shared);
+  if (factor) f = std::max(f, factor_lds_floats(P, false));
+  return (size_t)f * sizeof(float);
+}
+
+}  // namespace wmpc

@@ -79,8 +79,11 @@ def mass_chain_dynamics(n_masses: int = 6, nu: int = 4, springs=None, dt: float 
     if nu >= 1:
         force[0, 0] = 1.0
     if nu >= 2:
-        force[min(1, n_masses - 1), 1] += 1.0
-        force[min(2, n_masses - 1), 1] -= 1.0
+        if n_masses >= 3:  # pushes mass 1 against mass 2
+            force[1, 1] += 1.0
+            force[2, 1] -= 1.0
+        else:
+            force[n_masses - 1, 1] += 1.0
     if nu >= 3:
         force[max(n_masses - 3, 0), 2] += 1.0
         force[max(n_masses - 2, 0), 2] -= 1.0
@@ -120,6 +123,9 @@ def mpc_qp(
         ],
         format="csc",
     )
+    # kron of small diagonal blocks can store their off-diagonal zeros
+    p_matrix.eliminate_zeros()
+    p_matrix.sort_indices()
     ax = sp.kron(sp.eye(n_h + 1), -sp.eye(nx)) + sp.kron(sp.eye(n_h + 1, k=-1), sp.csc_matrix(ad))
     bu = sp.kron(sp.vstack([sp.csc_matrix((1, n_h)), sp.eye(n_h)]), sp.csc_matrix(bd))
     aeq = sp.hstack([ax, bu])
@@ -190,11 +196,58 @@ def batched_mpc_data(problem: LinearMPC, batch: int, *, seed: int = 0, x0_low=-1
     return p_values, a_values, q, l, u
 
 
+HETERO_SHAPES = tuple((nx, nu, n_h) for nx, nu in ((4, 2), (12, 4), (24, 8)) for n_h in (10, 20, 40))
+
+
+def hetero_mpc_patterns(seed: int = 2) -> list[LinearMPC]:
+    """The nine patterns of the heterogeneous-batch workload.
+
+    Mass chains with (nx, nu) in {(4, 2), (12, 4), (24, 8)} and horizon N in
+    {10, 20, 40}; per pattern, springs k ~ U[0.5, 2], Q = diag(U[0.1, 10]) and
+    R = diag(U[0.1, 1]) are drawn once; ZOH dt = 0.5, |x| <= 4, |u| <= 0.5.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for nx, nu, n_h in HETERO_SHAPES:
+        nm = nx // 2
+        ad, bd = mass_chain_dynamics(nm, nu, springs=rng.uniform(0.5, 2.0, nm + 1), dt=0.5)
+        out.append(mpc_qp(ad, bd, n_h, rng.uniform(0.1, 10.0, nx), rng.uniform(0.1, 1.0, nu),
+                          x_bound=4.0, u_bound=0.5))
+    return out
+
+
+def hetero_counts(total: int, patterns: int = 9, seed: int = 2) -> np.ndarray:
+    """Instances per pattern when `total` instances draw their pattern uniformly."""
+    rng = np.random.default_rng(seed + 1000)
+    return np.bincount(rng.integers(0, patterns, size=total), minlength=patterns)
+
+
+def mpc_shift_maps(problem: LinearMPC) -> tuple[np.ndarray, np.ndarray]:
+    """Source rows of the one-stage shift of an MPC solution (the previous
+    solution advanced one stage, the last stage repeated), for the mpc_qp
+    layout: variables [x_0..x_N, u_0..u_{N-1}], constraints [dynamics blocks
+    0..N (block k holds x_k's equation); identity rows of the variables].
+    Returns (variable map [n], constraint map [m]): shifted[i] = previous[map[i]].
+    """
+    nx, nu, n_h = problem.nx, problem.nu, problem.horizon
+    xs = np.minimum(np.arange(n_h + 1) + 1, n_h)
+    us = np.minimum(np.arange(n_h) + 1, n_h - 1)
+    xmap = (xs[:, None] * nx + np.arange(nx)).ravel()
+    umap = (n_h + 1) * nx + (us[:, None] * nu + np.arange(nu)).ravel()
+    var = np.concatenate([xmap, umap]).astype(np.int32)
+    con = np.concatenate([xmap, (n_h + 1) * nx + var]).astype(np.int32)
+    return var, con
+
+
 __all__ = [
+    "HETERO_SHAPES",
     "LinearMPC",
     "OSQP_INFTY",
     "batched_mpc_data",
     "cartpole_dynamics",
+    "hetero_counts",
+    "hetero_mpc_patterns",
+    "mpc_shift_maps",
     "kkt_microbench_plan",
     "kkt_microbench_values",
     "mass_chain_dynamics",
