@@ -16,6 +16,10 @@ Extra legs on the same JSON line:
   (N = 30, |u| <= 10, Q = diag(exp theta_q), R = exp theta_r per instance,
   batch 16,384): the forward solve, the loss cotangent, and one fused adjoint
   launch with the reference's 100 refinement steps;
+* ``hetero``: 200,000 instances over the nine mass-chain patterns
+  ((nx, nu) in {(4,2), (12,4), (24,8)} x N in {10, 20, 40}) in one
+  persistent launch, warm-started from the previous solution shifted one
+  stage;
 * ``qdldl``: the QDLDL factor + solve microbenchmark (mass chain N = 50,
   per-instance KKT values, batch 65,536);
 * ``osqp_per_instance``: the headline QP with per-instance P and A, so the
@@ -49,8 +53,8 @@ def _parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--legs", default="osqp,adjoint,qdldl,per_instance,cpu",
-                    help="comma-separated subset of osqp,adjoint,qdldl,per_instance,cpu")
+    ap.add_argument("--legs", default="osqp,adjoint,hetero,qdldl,per_instance,cpu",
+                    help="comma-separated subset of osqp,adjoint,hetero,qdldl,per_instance,cpu")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--refine", type=int, default=100, help="adjoint refinement steps")
     ap.add_argument("--ipw", type=int, default=0,
@@ -314,6 +318,68 @@ def _adjoint_leg(args, stream):
                 plan=plan, dp=dp, errs=_err_summary(errs), spread=_err_summary(spread), refine=refine)
 
 
+# --------------------------------------------------------------------------- heterogeneous batch (C2)
+
+
+def _hetero_leg(args, stream):
+    import numpy as np
+    import torch
+
+    from oracle import oracle
+    from warpmpc_amd.osqp import HeterogeneousOSQP, OSQPSettings, ShiftMaps, build_osqp_plan, compile_osqp, \
+        shift_warm_start
+    from warpmpc_amd.problems import batched_mpc_data, hetero_counts, hetero_mpc_patterns, mpc_shift_maps
+
+    total = 200_000
+    pats = hetero_mpc_patterns(seed=2)
+    counts = hetero_counts(total, seed=2)
+    _log(f"heterogeneous batch: 9 patterns, {total} instances {counts.tolist()}, 25 iterations, warm start = "
+         f"previous solution shifted one stage")
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plans = [build_osqp_plan(p.p_matrix, p.a_matrix, p.l, p.u, s) for p in pats]
+    het = HeterogeneousOSQP([compile_osqp(pl, "float32") for pl in plans])
+    f32 = dict(dtype=torch.float32, device="cuda")
+    data, sym, maps = [], [], []
+    for k, (p, pl, c) in enumerate(zip(pats, plans, counts)):
+        d = batched_mpc_data(p, int(c), seed=2 + k)
+        data.append(d)
+        t = lambda v: torch.as_tensor(np.ascontiguousarray(np.asarray(v).T), **f32)  # noqa: E731
+        sym.append(dict(p=t(d[0]), a=t(d[1]), q=t(d[2]), l=t(d[3]), u=t(d[4])))
+        maps.append(ShiftMaps(*mpc_shift_maps(p), pl.n, pl.m))
+    work = torch.empty((het.work_floats([int(c) for c in counts]),), **f32)
+    state = {"out": het.solve_symbolic(sym, work=work)}  # cold start; the timed steps warm-start
+    warm = [tuple(torch.empty_like(t) for t in o[:3]) for o in state["out"]]
+
+    def step():
+        for m_, o, w in zip(maps, state["out"], warm):
+            shift_warm_start(m_, o[0], o[1], o[2], out=w)
+        state["out"] = het.solve_symbolic(sym, warm_starts=warm, work=work)
+
+    wall, dev_ms = _timed(step, args.steps, args.warmup, stream)
+    # accuracy: sampled instances of the last step against the oracle from the same warm start
+    rng = np.random.default_rng(2)
+    errs = {"x": [], "z": [], "y": []}
+    for pl, d, o, w, c in zip(plans, data, state["out"], warm, counts):
+        cols = rng.choice(int(c), size=16, replace=False)
+        ws = [t[:, cols].t().cpu().numpy().astype(np.float64) for t in w]
+        ref = oracle.OracleOSQP(pl).solve(d[0], d[1], d[2][cols], d[3][cols], d[4][cols], x0=ws[0], z0=ws[1],
+                                          y0=ws[2])
+        for name, t, r in zip(("x", "z", "y"), o[:3], ref[:3]):
+            errs[name].append(_rel_err(t[:, cols].t().cpu().numpy(), r))
+    errs = {k: np.concatenate(v) for k, v in errs.items()}
+    # U4: sum over patterns of U2 (plus the warm start's x0, z0, y0 reads) per instance
+    u4 = sum(int(c) * (_u2_bytes(pl, 25) + S * (pl.n + 2 * pl.m)) for pl, c in zip(plans, counts))
+    ach = u4 / (dev_ms * 1e-3) / 1e9
+    return {"value": total * args.steps / wall, "unit": "QP solves/s", "instances": total,
+            "ms_per_step": wall * 1e3 / args.steps, "counts": counts.tolist(),
+            "config": "9 mass-chain patterns (nx,nu) in {(4,2),(12,4),(24,8)} x N in {10,20,40}, per-pattern "
+                      "springs/Q/R (seed 2), shared P/A per pattern, factor per instance, 25 iterations, warm "
+                      "start = previous solution shifted one stage (gather launches inside the step)",
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_step": u4, "kernel": "k_osqp_hetero"},
+            "err_vs_fp64": _err_summary(errs)}
+
+
 # --------------------------------------------------------------------------- QDLDL microbench (C1)
 
 
@@ -483,6 +549,8 @@ def main() -> int:
             "err_vs_fp64": r["errs"],
             "reference_fp32_spread": r["spread"],
         }
+    if "hetero" in legs:
+        line["hetero"] = _hetero_leg(args, stream)
     if "qdldl" in legs:
         line["qdldl"] = _qdldl_leg(args, stream)
     if "per_instance" in legs:
