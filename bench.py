@@ -20,6 +20,8 @@ Extra legs on the same JSON line:
   ((nx, nu) in {(4,2), (12,4), (24,8)} x N in {10, 20, 40}) in one
   persistent launch, warm-started from the previous solution shifted one
   stage;
+* ``sqp``: the nonlinear quadrotor MPC (N = 25, batch 100,000): 10 SQP
+  iterations with the filter line search, 25 ADMM iterations per QP;
 * ``qdldl``: the QDLDL factor + solve microbenchmark (mass chain N = 50,
   per-instance KKT values, batch 65,536);
 * ``osqp_per_instance``: the headline QP with per-instance P and A, so the
@@ -53,8 +55,8 @@ def _parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--legs", default="osqp,adjoint,hetero,qdldl,per_instance,cpu",
-                    help="comma-separated subset of osqp,adjoint,hetero,qdldl,per_instance,cpu")
+    ap.add_argument("--legs", default="osqp,adjoint,hetero,sqp,qdldl,per_instance,cpu",
+                    help="comma-separated subset of osqp,adjoint,hetero,sqp,qdldl,per_instance,cpu")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--refine", type=int, default=100, help="adjoint refinement steps")
     ap.add_argument("--ipw", type=int, default=0,
@@ -380,6 +382,77 @@ def _hetero_leg(args, stream):
             "err_vs_fp64": _err_summary(errs)}
 
 
+# --------------------------------------------------------------------------- nonlinear SQP (C4)
+
+
+def _sqp_leg(args, stream):
+    import numpy as np
+    import torch
+
+    from oracle import sqp_oracle
+    from warpmpc_amd.osqp import OSQPSettings
+    from warpmpc_amd.sqp import build_sparse_mpc_plan, compile_sparse_mpc_sqp
+    from warpmpc_amd.sqp.problems import quadrotor_initial_guess_and_params, quadrotor_problem
+
+    b, iters = 100_000, 10
+    prob = quadrotor_problem(horizon=25)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, scaling=0, rho_is_vec=True)
+    plan = build_sparse_mpc_plan(prob, osqp_settings=s)
+    sqp = compile_sparse_mpc_sqp(prob, plan, osqp_settings=s)
+    op = plan.osqp_plan
+    _log(f"nonlinear SQP: quadrotor N=25 (n={op.n}, m={op.m}, nnz_L={op.qdldl_plan.nnz_l}), batch {b}, "
+         f"{iters} line-search SQP iterations x 25 ADMM iterations")
+    z0, params = quadrotor_initial_guess_and_params(b, seed=4)
+    z0 = torch.as_tensor(z0, dtype=torch.float32, device="cuda")
+    params = torch.as_tensor(params, dtype=torch.float32, device="cuda")
+    phases = {"linearize": [], "qp": [], "line_search": []}
+
+    def solve_all(record=False):
+        z, state = z0, sqp.init_state(b)
+        for _ in range(iters):
+            if not record:
+                res, state = sqp.step(z, params, state=state)
+            else:
+                ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+                ev[0].record(stream)
+                lin = sqp.linearize_nlp(z, params)
+                ev[1].record(stream)
+                solve, state = sqp.solve_qp_data(lin, state=state)
+                ev[2].record(stream)
+                res = sqp.line_search_step(z, params, solve)
+                ev[3].record(stream)
+                torch.cuda.synchronize()
+                for k, (e0, e1) in zip(phases, zip(ev[:-1], ev[1:])):
+                    phases[k].append(e0.elapsed_time(e1))
+            z = res.z_next
+        return z
+
+    solves = max(1, args.steps // 5)
+    wall, _ = _timed(lambda: solve_all(), solves, 1, stream)
+    z_fin = solve_all(record=True)
+    # accuracy of one line-search step against the FP64 SQP oracle on a few instances
+    zs, ps = z0[:4].cpu().numpy(), params[:4].cpu().numpy()
+    res, _ = sqp.step(z0[:4], params[:4], state=sqp.init_state(4))
+    ref = sqp_oracle.OracleSQP(prob, plan).step(zs, ps)
+    err = _rel_err(res.solve.direction.cpu().numpy(), ref["direction"])
+    same_step = bool(np.array_equal(res.line_search.step_length.cpu().numpy(), ref["step_length"]))
+    ms = wall * 1e3 / solves
+    qp_ms = float(np.mean(phases["qp"]))
+    u2 = _u2_bytes(op, 25) + S * (op.n + 2 * op.m)  # warm-started QP
+    ach = u2 * b / (qp_ms * 1e-3) / 1e9
+    return {"value": b * solves / wall, "unit": "SQP solves/s (10 iterations each)", "qp_solves_per_s":
+            b * iters * solves / wall, "batch": b, "ms_per_solve": ms,
+            "phase_ms_per_iteration": {k: float(np.mean(v)) for k, v in phases.items()},
+            "finite": bool(torch.isfinite(z_fin).all()),
+            "config": "quadrotor (12 states, 4 thrusts, RK4 dt 0.02) N=25, x0 seed 4, hover initial guess, "
+                      "10 SQP iterations with the filter line search, 25 ADMM iterations per QP (warm start from "
+                      "the previous QP); linearization by torch.func on the GPU, QP by the HIP OSQP",
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": ach / HBM_PEAK_GBS, "kernel": "QP phase: k_osqp_admm (per-instance factor) + layout copies",
+                         "model_bytes_per_qp": u2, "kernel_ms": qp_ms},
+            "err_vs_fp64_first_step": {"direction": float(np.max(err)), "same_step_lengths": same_step}}
+
+
 # --------------------------------------------------------------------------- QDLDL microbench (C1)
 
 
@@ -551,6 +624,8 @@ def main() -> int:
         }
     if "hetero" in legs:
         line["hetero"] = _hetero_leg(args, stream)
+    if "sqp" in legs:
+        line["sqp"] = _sqp_leg(args, stream)
     if "qdldl" in legs:
         line["qdldl"] = _qdldl_leg(args, stream)
     if "per_instance" in legs:

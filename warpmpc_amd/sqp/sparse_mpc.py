@@ -159,6 +159,7 @@ class CompiledSparseMPCSQP:
     compute_direction: Callable
     fixed_step: Callable
     step: Callable
+    line_search_step: Callable
     line_search_settings: FilterLineSearchSettings
     line_search_step_lengths: np.ndarray
     extras: dict = field(default_factory=dict, compare=False, repr=False)
@@ -480,11 +481,11 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
         solve, nxt = compute_direction(z, params, state=state)
         return SQPStepResult(z_next=z + beta * solve.direction, solve=solve), nxt
 
-    def step(z, params, *, state: OSQPWarmStart | None = None):
-        """Line-search step (sparse_mpc.py:997-1053); returns (result, next state)."""
+    def line_search_step(z, params, solve: SQPSolveResult) -> SQPLineSearchStepResult:
+        """The filter line search along an already solved direction
+        (sparse_mpc.py:1011-1053)."""
         z = as_batch(z, n, "z")
         params = as_batch(params, plan.n_parameters, "params")
-        solve, nxt = solve_qp_data(linearize(z, params), state=state)
         base = solve.linearization
         d = solve.direction
         scale = ls.line_search_constraint_scale
@@ -500,7 +501,14 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
             armijo_descent_metric=metric, candidate_costs=cost.reshape(k, b), candidate_constraint_violations=viol)
         z_next = z + res.step_length[:, None] * d
         return SQPLineSearchStepResult(z_next=z_next, solve=solve, line_search=res,
-                                       is_finite=th.isfinite(z_next).all(dim=1)), nxt
+                                       is_finite=th.isfinite(z_next).all(dim=1))
+
+    def step(z, params, *, state: OSQPWarmStart | None = None):
+        """Line-search step (sparse_mpc.py:997-1053); returns (result, next state)."""
+        z = as_batch(z, n, "z")
+        params = as_batch(params, plan.n_parameters, "params")
+        solve, nxt = solve_qp_data(linearize(z, params), state=state)
+        return line_search_step(z, params, solve), nxt
 
     def evaluate_nlp(z, params):
         return evaluate_values(as_batch(z, n, "z"), as_batch(params, plan.n_parameters, "params"))
@@ -511,7 +519,8 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
     return CompiledSparseMPCSQP(
         plan=plan, qp_solver=qp_solver, osqp=qp, dtype=dtype, evaluate_nlp=evaluate_nlp,
         linearize_nlp=linearize_nlp, build_qp=linearize_nlp, init_state=init_state, solve_qp_data=solve_qp_data,
-        compute_direction=compute_direction, fixed_step=fixed_step, step=step, line_search_settings=ls,
+        compute_direction=compute_direction, fixed_step=fixed_step, step=step, line_search_step=line_search_step,
+        line_search_settings=ls,
         line_search_step_lengths=steps_np,
     )
 

@@ -45,6 +45,9 @@ class TorchStageFunction:
     hess_cols: np.ndarray
     jac_rows: np.ndarray
     jac_cols: np.ndarray
+    # cost-Hessian entries when equal at every probe (a quadratic cost): then
+    # they are used as constants and the Hessian is not differentiated out
+    hess_const: np.ndarray | None = None
 
     @classmethod
     def from_function(cls, fn: Callable, *, z_dim: int, param_dim: int, next_z_dim: int | None = None,
@@ -64,16 +67,20 @@ class TorchStageFunction:
         grad_nz = np.zeros(local, dtype=bool)
         hess_nz = np.zeros((local, local), dtype=bool)
         jac_nz = np.zeros((c, local), dtype=bool)
-        for _ in range(max(1, probes)):
+        hess_seen = []
+        for _ in range(max(2, probes)):
             loc = th.as_tensor(rng.standard_normal(local))
             p = th.as_tensor(rng.standard_normal(param_dim))
             grad, hess, jac = _derivatives(fn, has_next, int(z_dim), loc[None], p[None])
             grad_nz |= grad[0].numpy() != 0
             hess_nz |= hess[0].numpy() != 0
             jac_nz |= jac[0].numpy() != 0
+            hess_seen.append(hess[0].numpy())
         hess_nz |= hess_nz.T
         hr, hc = np.nonzero(np.triu(hess_nz))
         order = np.lexsort((hr, hc))  # column-major, like a CSC triplet walk
+        h_entries = np.stack([h[hr[order], hc[order]] for h in hess_seen])
+        hess_const = h_entries[0].copy() if np.all(h_entries == h_entries[0]) else None
         jr, jcol = np.nonzero(jac_nz)
         jorder = np.lexsort((jr, jcol))
         return cls(
@@ -89,6 +96,7 @@ class TorchStageFunction:
             hess_cols=hc[order].astype(np.int32),
             jac_rows=jr[jorder].astype(np.int32),
             jac_cols=jcol[jorder].astype(np.int32),
+            hess_const=hess_const,
         )
 
     def values(self, z, zn, p):
@@ -106,11 +114,15 @@ class TorchStageFunction:
         loc2 = loc.reshape(-1, loc.shape[-1])
         p2 = p.reshape(-1, p.shape[-1])
         cost, g, lower, upper = _call(self.fn, self.has_next, self.z_dim, loc2, p2)
-        grad, hess, jac = _derivatives(self.fn, self.has_next, self.z_dim, loc2, p2)
+        constant = self.hess_const is not None
+        grad, hess, jac = _derivatives(self.fn, self.has_next, self.z_dim, loc2, p2, hessian_too=not constant)
         dev = loc.device
         idx = lambda a: th.as_tensor(a, dtype=th.long, device=dev)  # noqa: E731
         ge = grad[:, idx(self.grad_cols)]
-        he = hess[:, idx(self.hess_rows), idx(self.hess_cols)]
+        if constant:
+            he = th.as_tensor(self.hess_const, dtype=loc.dtype, device=dev).expand(loc2.shape[0], -1)
+        else:
+            he = hess[:, idx(self.hess_rows), idx(self.hess_cols)]
         je = jac[:, idx(self.jac_rows), idx(self.jac_cols)]
         shape = lambda t: t.reshape(*lead, *t.shape[1:])  # noqa: E731
         return tuple(shape(t) for t in (cost, g, lower, upper, ge, he, je))
@@ -122,8 +134,9 @@ def _call(fn, has_next: bool, z_dim: int, loc, p):
     return fn(loc, p)
 
 
-def _derivatives(fn, has_next: bool, z_dim: int, loc, p):
-    """Dense cost gradient (B, L), cost Hessian (B, L, L) and g Jacobian (B, c, L)."""
+def _derivatives(fn, has_next: bool, z_dim: int, loc, p, hessian_too: bool = True):
+    """Dense cost gradient (B, L), cost Hessian (B, L, L) or None, and g
+    Jacobian (B, c, L)."""
     from torch.func import hessian, jacrev, vmap
 
     def cost1(x, q):
@@ -133,7 +146,7 @@ def _derivatives(fn, has_next: bool, z_dim: int, loc, p):
         return _call(fn, has_next, z_dim, x, q)[1]
 
     grad = vmap(jacrev(cost1))(loc, p)
-    hess = vmap(hessian(cost1))(loc, p)
+    hess = vmap(hessian(cost1))(loc, p) if hessian_too else None
     jac = vmap(jacrev(g1))(loc, p)
     return grad, hess, jac
 
