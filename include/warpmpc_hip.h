@@ -165,6 +165,26 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* desc, const wmpc_qdldl_plan* kkt
                           wmpc_osqp_plan** out);
 int wmpc_osqp_plan_destroy(wmpc_osqp_plan* plan);
 
+/* Straight-line kernels generated per pattern.  The reference's Warp backend
+ * generates its kernels per pattern at compile time (warp_backend.py:1175-1429,
+ * unrolled per-pattern loops); here a generator (warpmpc_amd/codegen.py) emits
+ * the whole ADMM iteration of a shared-factor solve as straight-line HIP code,
+ * compiled ahead of use into a gfx950 code object.
+ *
+ * wmpc_osqp_pattern_key: the 64-bit key of the pattern data such a kernel
+ * depends on (n, m, the KKT permutation and the structure of L); the
+ * generator names its kernel wmpc_admm_shared_<key as 16 hex digits>.
+ * wmpc_osqp_plan_specialize: loads a code object image (size bytes) on the
+ * current device and attaches its kernel for this plan's key; solves with a
+ * shared factor (factor_mode 1, or a passed factor with f_ld == 0) then launch
+ * it instead of the generic record-stream kernel.  An image without the key's
+ * kernel is rejected (an attached kernel then stays attached).  code_object ==
+ * NULL, size == 0 detaches.
+ * wmpc_osqp_plan_specialized: 1 when a kernel is attached, 0 otherwise. */
+int wmpc_osqp_pattern_key(const wmpc_osqp_plan* plan, uint64_t* key);
+int wmpc_osqp_plan_specialize(wmpc_osqp_plan* plan, const void* code_object, int64_t size);
+int wmpc_osqp_plan_specialized(const wmpc_osqp_plan* plan);
+
 /* CompiledOSQP.factor (solver.py:799-804): KKT value scatter + factorization.
  * p_values [nnz_p][pm_ld], a_values [nnz_a][pm_ld]; pm_ld = 0 means one matrix
  * shared by all instances (batch must then be 1 for the factor).  Outputs

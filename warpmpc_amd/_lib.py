@@ -160,6 +160,9 @@ SIGNATURES: dict[str, tuple] = {
                                                    _vp]),
     "wmpc_osqp_plan_create": (ctypes.c_int, [ctypes.POINTER(OsqpDesc), _vp, ctypes.POINTER(_vp)]),
     "wmpc_osqp_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_osqp_pattern_key": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "wmpc_osqp_plan_specialize": (ctypes.c_int, [_vp, _vp, _i64]),
+    "wmpc_osqp_plan_specialized": (ctypes.c_int, [_vp]),
     "wmpc_osqp_factor_f32": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, _i64, _vp]),
     "wmpc_osqp_work_floats": (ctypes.c_int64, [_vp, _i32, _i64]),
     "wmpc_osqp_solve_f32": (ctypes.c_int, [

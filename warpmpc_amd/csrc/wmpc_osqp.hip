@@ -86,6 +86,54 @@ static void group_by(int32_t groups, const std::vector<int32_t>& key, const std:
   }
 }
 
+// Key of the pattern data a generated ADMM kernel depends on: FNV-1a (64-bit)
+// over the little-endian bytes of the int32 words {generator version, n, m,
+// n_kkt, nnz_l}, p[n_kkt], lp[n_kkt + 1], li[nnz_l].  The generator
+// (warpmpc_amd/codegen.py) puts the same key into the kernel's name.
+constexpr int32_t kGenVersion = 1;
+
+static uint64_t fnv1a_words(uint64_t h, const int32_t* w, int64_t count) {
+  for (int64_t i = 0; i < count; ++i) {
+    const uint32_t v = (uint32_t)w[i];
+    for (int k = 0; k < 4; ++k) {
+      h ^= (v >> (8 * k)) & 0xffu;
+      h *= 0x100000001b3ull;
+    }
+  }
+  return h;
+}
+
+static uint64_t osqp_pattern_key(const wmpc_osqp_plan* plan) {
+  const wmpc_qdldl_pattern& pt = plan->kkt->pattern;
+  const int32_t head[5] = {kGenVersion, plan->d.n, plan->d.m, pt.n, pt.nnz_l};
+  uint64_t h = 0xcbf29ce484222325ull;
+  h = fnv1a_words(h, head, 5);
+  h = fnv1a_words(h, pt.p, pt.n);
+  h = fnv1a_words(h, pt.lp, pt.n + 1);
+  h = fnv1a_words(h, pt.li, pt.nnz_l);
+  return h;
+}
+
+// Launch of an attached straight-line kernel (the argument list of the
+// generated kernels, in order).
+static int launch_specialized(const wmpc_osqp_plan* plan, const float* p_values,
+                              const float* a_values, int64_t pm_ld, const float* q,
+                              const float* l, const float* u, const float* x0, const float* z0,
+                              const float* y0, const float* lx, const float* dinv, float* x,
+                              float* z, float* y, float* prim, float* dual, float* obj,
+                              int32_t max_iter, int64_t batch, int64_t ld, int32_t ipw,
+                              int64_t grid, hipStream_t st) {
+  OsqpDev dv = plan->d;
+  const float* vtab = reinterpret_cast<const float*>(dv.vtab);
+  const float* ctab = reinterpret_cast<const float*>(dv.ctab);
+  void* args[] = {&dv,  &p_values, &a_values, &pm_ld, &q,    &l,    &u,    &x0,
+                  &z0,  &y0,       &lx,       &dinv,  &x,    &z,    &y,    &prim,
+                  &dual, &obj,     &max_iter, &batch, &ld,   &ipw,  &vtab, &ctab};
+  WMPC_HIP_CHECK(hipModuleLaunchKernel((hipFunction_t)plan->spec_fn, (unsigned)grid, 1, 1, kWave,
+                                       1, 1, 0, st, args, nullptr));
+  return check_launch("osqp admm (generated)");
+}
+
 }  // namespace wmpc
 
 using namespace wmpc;
@@ -209,6 +257,7 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
 int wmpc_osqp_plan_destroy(wmpc_osqp_plan* plan) {
   clear_error();
   if (!plan) return WMPC_OK;
+  if (plan->spec_module) WMPC_HIP_CHECK(hipModuleUnload((hipModule_t)plan->spec_module));
   if (plan->block) WMPC_HIP_CHECK(hipFree(plan->block));
   delete plan;
   return WMPC_OK;
@@ -219,6 +268,58 @@ int64_t wmpc_osqp_work_floats(const wmpc_osqp_plan* plan, int32_t factor_mode, i
   const int64_t nk = plan->kkt->host.n, nl = plan->kkt->host.nnz_l;
   int64_t cols = factor_mode == 2 ? ld : (factor_mode == 1 ? 1 : 0);
   return 2 * nk * ld + (nl + nk) * cols;
+}
+
+int wmpc_osqp_pattern_key(const wmpc_osqp_plan* plan, uint64_t* key) {
+  clear_error();
+  WMPC_REQUIRE(plan && key, "null argument");
+  *key = osqp_pattern_key(plan);
+  return WMPC_OK;
+}
+
+int wmpc_osqp_plan_specialize(wmpc_osqp_plan* plan, const void* code_object, int64_t size) {
+  clear_error();
+  WMPC_REQUIRE(plan, "null argument");
+  WMPC_REQUIRE((code_object == nullptr) == (size == 0) && size >= 0,
+               "code_object and size must be given together (null and 0 detach)");
+  if (!code_object) {
+    if (plan->spec_module) {
+      WMPC_HIP_CHECK(hipModuleUnload((hipModule_t)plan->spec_module));
+      plan->spec_module = nullptr;
+      plan->spec_fn = nullptr;
+    }
+    return WMPC_OK;
+  }
+  // the new image replaces an attached one only once its kernel is found
+  hipModule_t mod = nullptr;
+  WMPC_HIP_CHECK(hipModuleLoadData(&mod, code_object));
+  char name[64];
+  std::snprintf(name, sizeof(name), "wmpc_admm_shared_%016llx",
+                (unsigned long long)osqp_pattern_key(plan));
+  hipFunction_t fn = nullptr;
+  if (hipModuleGetFunction(&fn, mod, name) != hipSuccess || !fn) {
+    (void)hipGetLastError();
+    (void)hipModuleUnload(mod);
+    set_error(std::string("code object has no kernel ") + name +
+              ": it was generated for another pattern (or generator version)");
+    return WMPC_ERR_INVALID;
+  }
+  if (plan->spec_module) {
+    hipError_t e = hipModuleUnload((hipModule_t)plan->spec_module);
+    plan->spec_module = mod;
+    plan->spec_fn = fn;
+    WMPC_HIP_CHECK(e);
+    return WMPC_OK;
+  }
+  plan->spec_module = mod;
+  plan->spec_fn = fn;
+  return WMPC_OK;
+}
+
+int wmpc_osqp_plan_specialized(const wmpc_osqp_plan* plan) {
+  clear_error();
+  WMPC_REQUIRE(plan, "null argument");
+  return plan->spec_fn ? 1 : 0;
 }
 
 int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
@@ -277,6 +378,9 @@ static int osqp_solve(const wmpc_osqp_plan* plan, const float* p_values, const f
     WMPC_REQUIRE(f_ld == 0 || f_ld >= batch, "f_ld must be 0 (shared) or >= batch");
     const bool sh = f_ld == 0;
     const size_t lds = admm_lds(P, sh, false);
+    if (sh && plan->spec_fn)
+      return launch_specialized(plan, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lx, dinv, x,
+                                z, y, prim, dual, obj, max_iter, batch, ld, ipw, grid, st);
     if (sh) {
       int rc = osqp_prepare_lds(k_osqp_admm<true, false>, lds);
       if (rc) return rc;
@@ -306,6 +410,9 @@ static int osqp_solve(const wmpc_osqp_plan* plan, const float* p_values, const f
                        (int64_t)1, (int64_t)1, kWave);
     rc = check_launch("osqp shared factor");
     if (rc) return rc;
+    if (plan->spec_fn)
+      return launch_specialized(plan, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, x,
+                                z, y, prim, dual, obj, max_iter, batch, ld, ipw, grid, st);
     const size_t lds = admm_lds(P, true, false);
     rc = osqp_prepare_lds(k_osqp_admm<true, false>, lds);
     if (rc) return rc;

@@ -45,6 +45,10 @@ struct wmpc_osqp_plan {
   wmpc::OsqpDev d;
   const wmpc_qdldl_plan* kkt = nullptr;
   void* block = nullptr;
+  // straight-line shared-factor ADMM kernel generated for this pattern
+  // (hipModule_t / hipFunction_t of an attached code object, or null)
+  void* spec_module = nullptr;
+  void* spec_fn = nullptr;
 };
 
 namespace wmpc {
@@ -99,58 +103,35 @@ __device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, 
   }
 }
 
-// One solve of instance b (solver.py:822-872) by this lane; every lane of the
-// wavefront takes part (the sweeps stage operands wave-wide).
-// SHARED: one factor for the batch ([nnz_l], [n_kkt]); FACTOR: assemble and
-// factor each instance's KKT into the work area first.
-template <bool SHARED, bool FACTOR>
-__device__ __forceinline__ void admm_solve(
-    const QdldlProg& P, const OsqpDev& dv, const float* __restrict__ pv,
-    const float* __restrict__ av, int64_t pm_ld, const float* __restrict__ q,
-    const float* __restrict__ l, const float* __restrict__ u, const float* __restrict__ x0,
-    const float* __restrict__ z0, const float* __restrict__ y0, float* __restrict__ lx,
-    float* __restrict__ dinv, int64_t f_ld, float* __restrict__ w, float* __restrict__ rbuf,
-    float* __restrict__ x, float* __restrict__ z, float* __restrict__ y, float* __restrict__ prim,
-    float* __restrict__ dual, float* __restrict__ obj, int32_t max_iter, int64_t ld, int64_t b,
-    float* lds, int lane) {
-  float* mywin = lds + (lane & (P.lcols - 1));
-  const int64_t pb = pm_ld ? b : 0;
-  const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
-  const int32_t n = dv.n, m = dv.m;
-  const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
-  if constexpr (FACTOR) {
-    KktSource src{pv, av, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-    factor_rows<false>(P, src, pb, Strided{lx, fs, fb}, Strided{nullptr, 0, 0},
-                       Strided{dinv, fs, fb}, mywin, NoRhs{}, Strided{nullptr, 0, 0});
-  }
-  const Strided X{x, ld, b}, Z{z, ld, b}, Y{y, ld, b}, R{rbuf, ld, b};
-  const CStrided Q{q, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, S{w, ld, b};
-  // scaled initial iterate (solver.py:842-847)
-  for (int32_t i = 0; i < n; ++i) X(i) = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
-  for (int32_t i = 0; i < m; ++i) {
+// Scaled initial iterate of instance b (solver.py:842-847).
+__device__ __forceinline__ void admm_initial(const OsqpDev& dv, const float* __restrict__ x0,
+                                             const float* __restrict__ z0,
+                                             const float* __restrict__ y0, const Strided& X,
+                                             const Strided& Z, const Strided& Y, int64_t ld,
+                                             int64_t b) {
+  for (int32_t i = 0; i < dv.n; ++i) X(i) = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
+  for (int32_t i = 0; i < dv.m; ++i) {
     Z(i) = z0 ? z0[(int64_t)i * ld + b] * dv.se[i] : 0.0f;
     Y(i) = y0 ? y0[(int64_t)i * ld + b] * dv.y_scale[i] : 0.0f;
   }
-  if (max_iter > 0) {
-    const SweepProg SP = sweep_prog(P);
-    admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
-    for (int32_t it = 0; it < max_iter; ++it) {
-      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, lds, lane);
-      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, lds, lane, StashOut{w, ld, b});
-      if (it + 1 < max_iter)
-        admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
-      else
-        admm_pass<false, true>(dv, S, R, X, Z, Y, Q, Lo, Up);
-    }
-  }
-  // unscale (solver.py:863-865)
+}
+
+// Unscaling (solver.py:863-865), then the residuals and the objective with the
+// unscaled P, A, q (solver.py:866-872), in gather form: no scatter scratch.
+__device__ __forceinline__ void admm_finish(const OsqpDev& dv, const float* __restrict__ pv,
+                                            const float* __restrict__ av, int64_t pm_ld,
+                                            const CStrided& Q, const Strided& X, const Strided& Z,
+                                            const Strided& Y, float* __restrict__ prim,
+                                            float* __restrict__ dual, float* __restrict__ obj,
+                                            int64_t b) {
+  const int64_t pb = pm_ld ? b : 0;
+  const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
+  const int32_t n = dv.n, m = dv.m;
   for (int32_t i = 0; i < n; ++i) X(i) *= dv.sd[i];
   for (int32_t i = 0; i < m; ++i) {
     Z(i) *= dv.seinv[i];
     Y(i) *= dv.y_unscale[i];
   }
-  // residuals and objective with the unscaled P, A, q (solver.py:866-872), in
-  // gather form: no scatter scratch
   float pr = 0.0f;
   for (int32_t i = 0; i < m; ++i) {
     float s = 0.0f;
@@ -179,6 +160,46 @@ __device__ __forceinline__ void admm_solve(
   prim[b] = pr;
   dual[b] = du;
   obj[b] = 0.5f * quad + lin;
+}
+
+// One solve of instance b (solver.py:822-872) by this lane; every lane of the
+// wavefront takes part (the sweeps stage operands wave-wide).
+// SHARED: one factor for the batch ([nnz_l], [n_kkt]); FACTOR: assemble and
+// factor each instance's KKT into the work area first.
+template <bool SHARED, bool FACTOR>
+__device__ __forceinline__ void admm_solve(
+    const QdldlProg& P, const OsqpDev& dv, const float* __restrict__ pv,
+    const float* __restrict__ av, int64_t pm_ld, const float* __restrict__ q,
+    const float* __restrict__ l, const float* __restrict__ u, const float* __restrict__ x0,
+    const float* __restrict__ z0, const float* __restrict__ y0, float* __restrict__ lx,
+    float* __restrict__ dinv, int64_t f_ld, float* __restrict__ w, float* __restrict__ rbuf,
+    float* __restrict__ x, float* __restrict__ z, float* __restrict__ y, float* __restrict__ prim,
+    float* __restrict__ dual, float* __restrict__ obj, int32_t max_iter, int64_t ld, int64_t b,
+    float* lds, int lane) {
+  float* mywin = lds + (lane & (P.lcols - 1));
+  const int64_t pb = pm_ld ? b : 0;
+  const int64_t fb = SHARED ? 0 : b, fs = SHARED ? 1 : f_ld;
+  if constexpr (FACTOR) {
+    KktSource src{pv, av, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
+    factor_rows<false>(P, src, pb, Strided{lx, fs, fb}, Strided{nullptr, 0, 0},
+                       Strided{dinv, fs, fb}, mywin, NoRhs{}, Strided{nullptr, 0, 0});
+  }
+  const Strided X{x, ld, b}, Z{z, ld, b}, Y{y, ld, b}, R{rbuf, ld, b};
+  const CStrided Q{q, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, S{w, ld, b};
+  admm_initial(dv, x0, z0, y0, X, Z, Y, ld, b);
+  if (max_iter > 0) {
+    const SweepProg SP = sweep_prog(P);
+    admm_pass<true, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+    for (int32_t it = 0; it < max_iter; ++it) {
+      fwd_cols<SHARED>(SP, lx, dinv, fs, fb, rbuf, w, ld, b, lds, lane);
+      bwd_cols<SHARED>(SP, lx, fs, fb, w, ld, b, lds, lane, StashOut{w, ld, b});
+      if (it + 1 < max_iter)
+        admm_pass<false, false>(dv, S, R, X, Z, Y, Q, Lo, Up);
+      else
+        admm_pass<false, true>(dv, S, R, X, Z, Y, Q, Lo, Up);
+    }
+  }
+  admm_finish(dv, pv, av, pm_ld, Q, X, Z, Y, prim, dual, obj, b);
 }
 
 // LDS bytes of one ADMM workgroup (sweeps, plus the factor windows when the

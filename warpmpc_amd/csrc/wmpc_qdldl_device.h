@@ -203,11 +203,17 @@ struct ArrayOut {
 };
 
 // NaN-propagating max/min with the semantics of jnp.maximum / jnp.minimum.
+// Select form: both operands are always used, so the compiler cannot sink the
+// computation of one of them (a load, say) into a branch.
 __device__ __forceinline__ float nan_max(float a, float b) {
-  return (a != a) ? a : ((b != b) ? b : fmaxf(a, b));
+  float r = fmaxf(a, b);
+  r = (b != b) ? b : r;
+  return (a != a) ? a : r;
 }
 __device__ __forceinline__ float nan_min(float a, float b) {
-  return (a != a) ? a : ((b != b) ? b : fminf(a, b));
+  float r = fminf(a, b);
+  r = (b != b) ? b : r;
+  return (a != a) ? a : r;
 }
 
 }  // namespace wmpc

@@ -65,6 +65,29 @@ class DeviceOSQPPlan:
                    "wmpc_osqp_plan_create")
         self.handle = handle
 
+    def pattern_key(self) -> int:
+        key = ctypes.c_uint64()
+        _lib.check(_lib.load().wmpc_osqp_pattern_key(self.handle, ctypes.byref(key)), "wmpc_osqp_pattern_key")
+        return int(key.value)
+
+    def specialize(self, code_object: bytes | None) -> None:
+        """Attaches a generated straight-line kernel (code object bytes), or
+        detaches with None."""
+        lib = _lib.load()
+        if code_object is None:
+            _lib.check(lib.wmpc_osqp_plan_specialize(self.handle, None, 0), "wmpc_osqp_plan_specialize")
+            return
+        buf = ctypes.create_string_buffer(code_object, len(code_object))
+        _lib.check(lib.wmpc_osqp_plan_specialize(self.handle, ctypes.cast(buf, ctypes.c_void_p), len(code_object)),
+                   "wmpc_osqp_plan_specialize")
+
+    @property
+    def specialized(self) -> bool:
+        r = int(_lib.load().wmpc_osqp_plan_specialized(self.handle))
+        if r < 0:
+            _lib.check(r, "wmpc_osqp_plan_specialized")
+        return r == 1
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -106,8 +129,15 @@ def compile_osqp(
     level_scheduled_solve_threshold: int = 1,
     derivatives: bool = False,
     derivative_refinement_iters: int = 100,
+    specialize: bool | str = "auto",
 ) -> CompiledOSQP:
-    """Compile fixed-pattern batched OSQP on the HIP backend (``solver.py:685-737``)."""
+    """Compile fixed-pattern batched OSQP on the HIP backend (``solver.py:685-737``).
+
+    ``specialize`` (HIP only): shared-factor solves of a pattern with a
+    generated straight-line kernel (``warpmpc_amd.codegen``) run that kernel.
+    ``"auto"`` attaches the code object when one was built for this pattern,
+    ``True`` builds it first if needed (hipcc), ``False`` keeps the generic
+    kernel."""
     dtype = np.dtype(dtype)
     settings = plan.settings
     qd = compile_qdldl_variant(
@@ -129,6 +159,14 @@ def compile_osqp(
         )
     require_gpu()
     dev = DeviceOSQPPlan(plan, qd.extras["device_plan"])
+    if specialize not in (True, False, "auto"):
+        raise ValueError("specialize must be True, False or 'auto'")
+    if specialize:
+        from .. import codegen
+
+        path = codegen.build_admm_kernel(plan) if specialize is True else codegen.find_admm_kernel(plan)
+        if path is not None:
+            dev.specialize(path.read_bytes())
     lib = _lib.load()
     th = torch()
     n, m = plan.n, plan.m
