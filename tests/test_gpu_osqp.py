@@ -87,6 +87,52 @@ def test_warm_start_and_factor_reuse(gpu, chain):
         np.testing.assert_allclose(a.cpu().numpy(), c.cpu().numpy(), rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("horizon", [1, 10, 40])
+def test_shared_factor_wavefront_equals_lane_factor(gpu, horizon):
+    """One matrix for the batch is factored by one wavefront (the reach
+    updates spread over the lanes); a one-instance per-instance launch factors
+    it in one lane.  Same operations in the same order per value: bitwise
+    equal L and D^-1."""
+    from warpmpc_amd import _lib
+
+    th = gpu
+    prob = mass_chain_mpc(horizon=horizon)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
+    solver = compile_osqp(plan, np.float32, specialize=False)
+    dev = solver.extras["device_plan"]
+    lib = _lib.load()
+    pv, av, *_ = batched_mpc_data(prob, 1, seed=horizon)
+    rng = np.random.default_rng(horizon)
+    pv = pv * rng.uniform(0.5, 2.0, size=pv.shape)
+    f32 = dict(dtype=th.float32, device="cuda")
+    p_t = th.as_tensor(pv[0], **f32)
+    a_t = th.as_tensor(av[0], **f32)
+    nl, nk = plan.qdldl_plan.nnz_l, plan.qdldl_plan.n
+    out = {}
+    for name, pm_ld in (("wave", 0), ("lane", 1)):
+        lx = th.full((nl,), np.nan, **f32)
+        dinv = th.full((nk,), np.nan, **f32)
+        print(f"shared factor N={horizon} via {name}", flush=True)
+        _lib.check(lib.wmpc_osqp_factor_f32(dev.handle, p_t.data_ptr(), a_t.data_ptr(), pm_ld, lx.data_ptr(),
+                                            dinv.data_ptr(), 1, 1, None), "wmpc_osqp_factor_f32")
+        th.cuda.synchronize()
+        out[name] = (lx.cpu().numpy(), dinv.cpu().numpy())
+    np.testing.assert_array_equal(out["wave"][0], out["lane"][0])
+    np.testing.assert_array_equal(out["wave"][1], out["lane"][1])
+    _, _, dinv64 = oracle.qdldl_factor(plan.qdldl_plan, _kkt_values(plan, pv[0], av[0])[None, :])
+    assert np.max(np.abs(out["wave"][1] - dinv64[0]) / np.abs(dinv64[0])) < 1e-4
+
+
+def _kkt_values(plan, pv, av):
+    """Original-order KKT values of one instance: base values, P (+ sigma on
+    its diagonal) and A, with the plan's scaling."""
+    vals = np.asarray(plan.kkt_upper.data, dtype=np.float64).copy()
+    vals[plan.p_to_kkt] = pv * plan.p_scale + plan.p_diag_sigma
+    vals[plan.a_to_kkt] = av * plan.a_scale
+    return vals
+
+
 def test_reference_fixture_scaled_problem(gpu):
     """tests/test_jax_osqp.py:97-125 on the HIP backend (float32).
 

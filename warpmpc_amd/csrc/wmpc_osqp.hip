@@ -55,6 +55,63 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
                      Strided{nullptr, 0, 0});
 }
 
+// Factorization of one matrix shared by the whole batch by one wavefront.
+// Row k of the up-looking recurrence (core.py:867-933): scatter the upper
+// entries of column k into the row workspace Y, then for each reach entry c in
+// processing order the updates Y[li[q]] -= L[q] Y[c] over the column's entries
+// before row k -- distinct rows, one lane each -- and L[k,c] = Y[c] D⁻¹[c],
+// D[k] -= Y[c] L[k,c].  Every value sees the same operations in the same order
+// as in the one-lane factorization; L, its row indices, D⁻¹ and Y stay in LDS.
+// The wavefront's LDS accesses execute in program order, so no barrier is
+// needed between dependent steps.
+__global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F, KktSource src,
+                                                              float* __restrict__ lx,
+                                                              float* __restrict__ dinv) {
+  extern __shared__ float sm[];
+  const int lane = threadIdx.x;
+  const int32_t n = F.n, nl = F.nnz_l;
+  float* Y = sm;
+  float* Di = Y + n;
+  float* Lx = Di + n;
+  int32_t* Li = reinterpret_cast<int32_t*>(Lx + nl);
+  for (int32_t i = lane; i < n; i += kWave) Y[i] = 0.f;
+  for (int32_t q = lane; q < nl; q += kWave) Li[q] = F.li[q];
+  __syncthreads();
+  const int32_t last = F.nsteps > 0 ? F.nsteps - 1 : 0;
+  int32_t r = 0;
+  int4 next = F.nsteps > 0 ? F.steps[0] : int4{0, 0, 0, 0};
+  for (int32_t k = 0; k < n; ++k) {
+    for (int32_t j = F.init_ptr[k] + lane; j < F.init_ptr[k + 1]; j += kWave) {
+      const int2 e = F.init[j];
+      Y[e.x] = src(e.y, 0);
+    }
+    float dk = src(F.diag[k], 0);
+    const int32_t r1 = F.row_ptr[k + 1];
+    for (; r < r1; ++r) {
+      const int4 st = next;  // {c, lidx, count, lp[c]}
+      next = F.steps[r + 1 < last ? r + 1 : last];
+      const float yc = Y[st.x];
+      for (int32_t t = lane; t < st.z; t += kWave) {
+        const int32_t q = st.w + t;
+        Y[Li[q]] -= Lx[q] * yc;
+      }
+      const float lval = yc * Di[st.x];
+      if (lane == 0) {
+        Lx[st.y] = lval;
+        Y[st.x] = 0.f;
+      }
+      dk -= yc * lval;
+    }
+    const float di = 1.0f / dk;
+    if (lane == 0) {
+      Di[k] = di;
+      dinv[k] = di;
+    }
+  }
+  __syncthreads();
+  for (int32_t q = lane; q < nl; q += kWave) lx[q] = Lx[q];
+}
+
 static size_t factor_lds(const QdldlProg& P) {
   return (size_t)factor_lds_floats(P, false) * sizeof(float);
 }
@@ -134,6 +191,27 @@ static int launch_specialized(const wmpc_osqp_plan* plan, const float* p_values,
   return check_launch("osqp admm (generated)");
 }
 
+// Factorization of one matrix shared by the batch: by one wavefront when its
+// LDS fits, else by the one-lane row-program kernel.
+static int factor_shared(const wmpc_osqp_plan* plan, const KktSource& src, float* lx, float* dinv,
+                         hipStream_t st) {
+  const size_t lds = shared_factor_lds(plan->sf);
+  if (lds <= kMaxLdsBytes) {
+    int rc = osqp_prepare_lds(k_osqp_factor_shared, lds);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_osqp_factor_shared, dim3(1), dim3(kWave), lds, st, plan->sf, src, lx, dinv);
+    return check_launch("osqp shared factor");
+  }
+  const QdldlProg& P = plan->kkt->prog;
+  const size_t flds = factor_lds(P);
+  int rc = osqp_prepare_lds(k_osqp_factor, flds);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_osqp_factor, dim3(1), dim3(kWave), flds, st, P, src, lx, dinv, (int64_t)1,
+                     (int64_t)1, kWave);
+  return check_launch("osqp shared factor");
+}
+
+
 }  // namespace wmpc
 
 using namespace wmpc;
@@ -208,7 +286,27 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
                                float_bits(ds->scaling_e[i]), 0, 0});
     }
   }
+  // one-wavefront shared factorization
+  const wmpc_qdldl_pattern& pt = kkt->pattern;
+  const int32_t nk = pt.n, nproc = pt.proc_ptr[nk];
+  std::vector<int32_t> sf_steps(4 * (size_t)std::max(nproc, 1), 0), sf_init(2 * (size_t)std::max(pt.init_ptr[nk], 1), 0),
+      sf_diag(nk), sf_rowptr(pt.proc_ptr, pt.proc_ptr + nk + 1), sf_initptr(pt.init_ptr, pt.init_ptr + nk + 1),
+      sf_li(pt.li, pt.li + std::max(pt.nnz_l, 1));
+  for (int32_t r = 0; r < nproc; ++r) {
+    const int32_t c = pt.proc_cols[r];
+    sf_steps[4 * (size_t)r] = c;
+    sf_steps[4 * (size_t)r + 1] = pt.proc_lidx[r];
+    sf_steps[4 * (size_t)r + 2] = pt.proc_off[r];
+    sf_steps[4 * (size_t)r + 3] = pt.lp[c];
+  }
+  for (int32_t j = 0; j < pt.init_ptr[nk]; ++j) {
+    sf_init[2 * (size_t)j] = pt.init_rows[j];
+    sf_init[2 * (size_t)j + 1] = pt.perm_to_orig[pt.init_pos[j]];
+  }
+  for (int32_t k = 0; k < nk; ++k) sf_diag[k] = pt.perm_to_orig[pt.diag_pos[k]];
   Blob blob;
+  const size_t t_sfs = blob.add(sf_steps), t_sfi = blob.add(sf_init), t_sfd = blob.add(sf_diag),
+               t_sfr = blob.add(sf_rowptr), t_sfp = blob.add(sf_initptr), t_sfl = blob.add(sf_li);
   const size_t t_kind = blob.add(kind), t_idx = blob.add(idx), t_scale = blob.add(scale),
                t_off = blob.add(off), t_vtab = blob.add(vtab), t_ctab = blob.add(ctab);
   const size_t t_d = blob.add_f32(ds->scaling_d, n), t_dinv = blob.add_f32(ds->scaling_dinv, n),
@@ -249,6 +347,16 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   d.acol = blob.at<int2>(t_ac);
   d.prow_ptr = blob.at<int32_t>(t_prp);
   d.prow = blob.at<int2>(t_pr);
+  SharedFactorProg& sf = plan->sf;
+  sf.n = nk;
+  sf.nnz_l = pt.nnz_l;
+  sf.nsteps = nproc;
+  sf.steps = blob.at<int4>(t_sfs);
+  sf.init = blob.at<int2>(t_sfi);
+  sf.diag = blob.at<int32_t>(t_sfd);
+  sf.row_ptr = blob.at<int32_t>(t_sfr);
+  sf.init_ptr = blob.at<int32_t>(t_sfp);
+  sf.li = blob.at<int32_t>(t_sfl);
   plan->kkt = kkt;
   *out = plan;
   return WMPC_OK;
@@ -333,11 +441,12 @@ int wmpc_osqp_factor_f32(const wmpc_osqp_plan* plan, const float* p_values,
   if (batch == 0) return WMPC_OK;
   const QdldlProg& P = plan->kkt->prog;
   const OsqpDev& dv = plan->d;
+  KktSource src{p_values, a_values, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
+  if (pm_ld == 0 && ld == 1) return factor_shared(plan, src, lx, dinv, (hipStream_t)stream);
   const size_t lds = factor_lds(P);
   int rc = osqp_prepare_lds(k_osqp_factor, lds);
   if (rc) return rc;
   const int32_t ipw = instances_per_wave(batch, P);
-  KktSource src{p_values, a_values, pm_ld, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
   hipLaunchKernelGGL(k_osqp_factor, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, P, src, lx, dinv, batch, ld, ipw);
   return check_launch("osqp factor");
@@ -403,12 +512,7 @@ static int osqp_solve(const wmpc_osqp_plan* plan, const float* p_values, const f
     float* lxw = fwork;
     float* dinvw = lxw + nl;
     KktSource src{p_values, a_values, 0, dv.src_kind, dv.src_idx, dv.src_scale, dv.src_off};
-    const size_t flds = factor_lds(P);
-    int rc = osqp_prepare_lds(k_osqp_factor, flds);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_osqp_factor, dim3(1), dim3(kWave), flds, st, P, src, lxw, dinvw,
-                       (int64_t)1, (int64_t)1, kWave);
-    rc = check_launch("osqp shared factor");
+    int rc = factor_shared(plan, src, lxw, dinvw, st);
     if (rc) return rc;
     if (plan->spec_fn)
       return launch_specialized(plan, p_values, a_values, pm_ld, q, l, u, x0, z0, y0, lxw, dinvw, x,

@@ -39,10 +39,29 @@ struct OsqpDev {
   const int2* prow = nullptr;         // {col, P value slot}
 };
 
+// Tables of the one-wavefront factorization of a matrix shared by the batch
+// (k_osqp_factor_shared): the up-looking recurrence of QDLDL with the updates
+// of each reach entry spread over the lanes.
+struct SharedFactorProg {
+  int32_t n = 0, nnz_l = 0, nsteps = 0;
+  const int32_t* row_ptr = nullptr;   // [n+1] reach entries of row k: [row_ptr[k], row_ptr[k+1])
+  const int4* steps = nullptr;        // [nsteps] {c, L slot of L[k,c], entries of column c before row k, lp[c]}
+  const int32_t* init_ptr = nullptr;  // [n+1]
+  const int2* init = nullptr;         // {row, original KKT slot}: upper entries of column k
+  const int32_t* diag = nullptr;      // [n] original KKT slot of K[k,k]
+  const int32_t* li = nullptr;        // [nnz_l] L row indices
+};
+
+// LDS bytes of k_osqp_factor_shared: row workspace, D⁻¹, L values and rows.
+inline size_t shared_factor_lds(const SharedFactorProg& F) {
+  return (size_t)(2 * (int64_t)F.n + 2 * (int64_t)F.nnz_l) * sizeof(float);
+}
+
 }  // namespace wmpc
 
 struct wmpc_osqp_plan {
   wmpc::OsqpDev d;
+  wmpc::SharedFactorProg sf;
   const wmpc_qdldl_plan* kkt = nullptr;
   void* block = nullptr;
   // straight-line shared-factor ADMM kernel generated for this pattern
