@@ -61,52 +61,89 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
 // processing order the updates Y[li[q]] -= L[q] Y[c] over the column's entries
 // before row k -- distinct rows, one lane each -- and L[k,c] = Y[c] D⁻¹[c],
 // D[k] -= Y[c] L[k,c].  Every value sees the same operations in the same order
-// as in the one-lane factorization; L, its row indices, D⁻¹ and Y stay in LDS.
-// The wavefront's LDS accesses execute in program order, so no barrier is
-// needed between dependent steps.
+// as in the one-lane factorization.  All tables and values live in LDS (the
+// KKT values are assembled there by all lanes first), so the loop touches no
+// global memory until L is written out.  A step's record is read two steps
+// ahead and each lane's L entry and target row one step ahead (L entries of a
+// reach column before row k were written in earlier rows; at the start of a
+// row they are read again, since the row just finished may have written them);
+// lanes past a step's update count work on a dummy row.  The wavefront's LDS
+// accesses execute in program order, so dependent steps need no barrier.
 __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F, KktSource src,
                                                               float* __restrict__ lx,
                                                               float* __restrict__ dinv) {
   extern __shared__ float sm[];
   const int lane = threadIdx.x;
-  const int32_t n = F.n, nl = F.nnz_l;
-  float* Y = sm;
-  float* Di = Y + n;
-  float* Lx = Di + n;
-  int32_t* Li = reinterpret_cast<int32_t*>(Lx + nl);
-  for (int32_t i = lane; i < n; i += kWave) Y[i] = 0.f;
-  for (int32_t q = lane; q < nl; q += kWave) Li[q] = F.li[q];
+  const int32_t n = F.n, nl = F.nnz_l, ns = F.nsteps;
+  int4* Rw = reinterpret_cast<int4*>(sm);
+  int2* In = reinterpret_cast<int2*>(sm + 4 * (int64_t)n);
+  int2* St = In + F.ninit;
+  float* Y = reinterpret_cast<float*>(St + ns);  // [n + 1]
+  float* Di = Y + n + 1;
+  float* Lx = Di + n;                                  // [nl + 1]
+  int32_t* Li = reinterpret_cast<int32_t*>(Lx + nl + 1);  // [nl + 1]
+  float* Kv = reinterpret_cast<float*>(Li + nl + 1);
+  for (int32_t i = lane; i < n; i += kWave) Rw[i] = F.rows[i];
+  for (int32_t i = lane; i <= n; i += kWave) Y[i] = 0.f;
+  for (int32_t j = lane; j < F.ninit; j += kWave) In[j] = F.init[j];
+  for (int32_t q = lane; q <= nl; q += kWave) {
+    Li[q] = q < nl ? F.li[q] : 0;
+    Lx[q] = 0.f;
+  }
+  for (int32_t r = lane; r < ns; r += kWave) St[r] = F.steps[r];
+  for (int32_t s = lane; s < F.nnz_kkt; s += kWave) Kv[s] = src(s, 0);
   __syncthreads();
-  const int32_t last = F.nsteps > 0 ? F.nsteps - 1 : 0;
+  // this lane's update of a step: target row (n: the dummy row) and L value
+  auto lane_update = [&](int2 rec, int32_t& row, float& lv) {
+    const int32_t cnt = rec.x >> 16;
+    const bool on = lane < cnt;
+    const int32_t q = rec.y + (on ? lane : 0);  // <= nnz_l (a padding slot)
+    const int32_t li = Li[q];
+    const float l = Lx[q];
+    row = on ? li : n;
+    lv = on ? l : 0.f;
+  };
+  const int32_t slast = ns > 0 ? ns - 1 : 0;
+  int2 rec_a = ns > 0 ? St[0] : int2{0, 0};
+  int2 rec_b = ns > 0 ? St[ns > 1 ? 1 : 0] : int2{0, 0};
   int32_t r = 0;
-  int4 next = F.nsteps > 0 ? F.steps[0] : int4{0, 0, 0, 0};
   for (int32_t k = 0; k < n; ++k) {
-    for (int32_t j = F.init_ptr[k] + lane; j < F.init_ptr[k + 1]; j += kWave) {
-      const int2 e = F.init[j];
-      Y[e.x] = src(e.y, 0);
+    const int4 row = Rw[k];  // {end of steps, init begin, init end, diagonal slot}
+    for (int32_t j = row.y + lane; j < row.z; j += kWave) {
+      const int2 e = In[j];
+      Y[e.x] = Kv[e.y];
     }
-    float dk = src(F.diag[k], 0);
-    const int32_t r1 = F.row_ptr[k + 1];
-    for (; r < r1; ++r) {
-      const int4 st = next;  // {c, lidx, count, lp[c]}
-      next = F.steps[r + 1 < last ? r + 1 : last];
-      const float yc = Y[st.x];
-      for (int32_t t = lane; t < st.z; t += kWave) {
-        const int32_t q = st.w + t;
-        Y[Li[q]] -= Lx[q] * yc;
+    float dk = Kv[row.w];
+    int32_t ia = n;
+    float la = 0.f;
+    if (ns > 0) lane_update(rec_a, ia, la);
+    for (; r < row.x; ++r) {
+      const int32_t sx = __builtin_amdgcn_readfirstlane(rec_a.x);
+      const int32_t q0 = __builtin_amdgcn_readfirstlane(rec_a.y);
+      const int32_t c = sx & 0xffff, cnt = sx >> 16;
+      const int32_t i0 = ia;
+      const float l0 = la;
+      lane_update(rec_b, ia, la);  // step r + 1 (its row is k or starts afresh)
+      rec_a = rec_b;
+      rec_b = St[r + 2 < ns ? r + 2 : slast];
+      const float yc = Y[c];
+      const float dc = Di[c];
+      const float yi = Y[i0];
+      Y[i0] = yi - l0 * yc;
+      if (cnt > kWave) {
+        for (int32_t t = lane + kWave; t < cnt; t += kWave) {
+          const int32_t q = q0 + t;
+          Y[Li[q]] -= Lx[q] * yc;
+        }
       }
-      const float lval = yc * Di[st.x];
-      if (lane == 0) {
-        Lx[st.y] = lval;
-        Y[st.x] = 0.f;
-      }
+      const float lval = yc * dc;
+      Lx[q0 + cnt] = lval;  // every lane stores the same value
+      Y[c] = 0.f;
       dk -= yc * lval;
     }
     const float di = 1.0f / dk;
-    if (lane == 0) {
-      Di[k] = di;
-      dinv[k] = di;
-    }
+    Di[k] = di;
+    if (lane == 0) dinv[k] = di;
   }
   __syncthreads();
   for (int32_t q = lane; q < nl; q += kWave) lx[q] = Lx[q];
@@ -196,7 +233,7 @@ static int launch_specialized(const wmpc_osqp_plan* plan, const float* p_values,
 static int factor_shared(const wmpc_osqp_plan* plan, const KktSource& src, float* lx, float* dinv,
                          hipStream_t st) {
   const size_t lds = shared_factor_lds(plan->sf);
-  if (lds <= kMaxLdsBytes) {
+  if (plan->sf.n > 0 && lds <= kMaxLdsBytes) {
     int rc = osqp_prepare_lds(k_osqp_factor_shared, lds);
     if (rc) return rc;
     hipLaunchKernelGGL(k_osqp_factor_shared, dim3(1), dim3(kWave), lds, st, plan->sf, src, lx, dinv);
@@ -286,27 +323,31 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
                                float_bits(ds->scaling_e[i]), 0, 0});
     }
   }
-  // one-wavefront shared factorization
+  // one-wavefront shared factorization (steps pack c and the update count in
+  // 16 bits each: larger patterns keep the one-lane kernel)
   const wmpc_qdldl_pattern& pt = kkt->pattern;
-  const int32_t nk = pt.n, nproc = pt.proc_ptr[nk];
-  std::vector<int32_t> sf_steps(4 * (size_t)std::max(nproc, 1), 0), sf_init(2 * (size_t)std::max(pt.init_ptr[nk], 1), 0),
-      sf_diag(nk), sf_rowptr(pt.proc_ptr, pt.proc_ptr + nk + 1), sf_initptr(pt.init_ptr, pt.init_ptr + nk + 1),
-      sf_li(pt.li, pt.li + std::max(pt.nnz_l, 1));
-  for (int32_t r = 0; r < nproc; ++r) {
-    const int32_t c = pt.proc_cols[r];
-    sf_steps[4 * (size_t)r] = c;
-    sf_steps[4 * (size_t)r + 1] = pt.proc_lidx[r];
-    sf_steps[4 * (size_t)r + 2] = pt.proc_off[r];
-    sf_steps[4 * (size_t)r + 3] = pt.lp[c];
+  const int32_t nk = pt.n, nproc = pt.proc_ptr[nk], ninit = pt.init_ptr[nk];
+  std::vector<int32_t> sf_rows(4 * (size_t)nk), sf_steps(2 * (size_t)std::max(nproc, 1), 0),
+      sf_init(2 * (size_t)std::max(ninit, 1), 0), sf_li(pt.li, pt.li + std::max(pt.nnz_l, 1));
+  bool sf_ok = nk < 65536;
+  for (int32_t k = 0; k < nk; ++k) {
+    sf_rows[4 * (size_t)k] = pt.proc_ptr[k + 1];
+    sf_rows[4 * (size_t)k + 1] = pt.init_ptr[k];
+    sf_rows[4 * (size_t)k + 2] = pt.init_ptr[k + 1];
+    sf_rows[4 * (size_t)k + 3] = pt.perm_to_orig[pt.diag_pos[k]];
   }
-  for (int32_t j = 0; j < pt.init_ptr[nk]; ++j) {
+  for (int32_t r = 0; r < nproc; ++r) {
+    sf_ok = sf_ok && pt.proc_off[r] < 32768;
+    sf_steps[2 * (size_t)r] = pt.proc_cols[r] | (pt.proc_off[r] << 16);
+    sf_steps[2 * (size_t)r + 1] = pt.lp[pt.proc_cols[r]];
+  }
+  for (int32_t j = 0; j < ninit; ++j) {
     sf_init[2 * (size_t)j] = pt.init_rows[j];
     sf_init[2 * (size_t)j + 1] = pt.perm_to_orig[pt.init_pos[j]];
   }
-  for (int32_t k = 0; k < nk; ++k) sf_diag[k] = pt.perm_to_orig[pt.diag_pos[k]];
   Blob blob;
-  const size_t t_sfs = blob.add(sf_steps), t_sfi = blob.add(sf_init), t_sfd = blob.add(sf_diag),
-               t_sfr = blob.add(sf_rowptr), t_sfp = blob.add(sf_initptr), t_sfl = blob.add(sf_li);
+  const size_t t_sfr = blob.add(sf_rows), t_sfs = blob.add(sf_steps), t_sfl = blob.add(sf_li),
+               t_sfi = blob.add(sf_init);
   const size_t t_kind = blob.add(kind), t_idx = blob.add(idx), t_scale = blob.add(scale),
                t_off = blob.add(off), t_vtab = blob.add(vtab), t_ctab = blob.add(ctab);
   const size_t t_d = blob.add_f32(ds->scaling_d, n), t_dinv = blob.add_f32(ds->scaling_dinv, n),
@@ -351,12 +392,13 @@ int wmpc_osqp_plan_create(const wmpc_osqp_desc* ds, const wmpc_qdldl_plan* kkt,
   sf.n = nk;
   sf.nnz_l = pt.nnz_l;
   sf.nsteps = nproc;
-  sf.steps = blob.at<int4>(t_sfs);
-  sf.init = blob.at<int2>(t_sfi);
-  sf.diag = blob.at<int32_t>(t_sfd);
-  sf.row_ptr = blob.at<int32_t>(t_sfr);
-  sf.init_ptr = blob.at<int32_t>(t_sfp);
+  sf.nnz_kkt = ds->nnz_kkt;
+  sf.ninit = ninit;
+  sf.rows = blob.at<int4>(t_sfr);
+  sf.steps = blob.at<int2>(t_sfs);
   sf.li = blob.at<int32_t>(t_sfl);
+  sf.init = blob.at<int2>(t_sfi);
+  if (!sf_ok) sf.n = 0;  // marks the one-lane fallback
   plan->kkt = kkt;
   *out = plan;
   return WMPC_OK;

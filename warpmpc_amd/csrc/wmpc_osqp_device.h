@@ -41,20 +41,31 @@ struct OsqpDev {
 
 // Tables of the one-wavefront factorization of a matrix shared by the batch
 // (k_osqp_factor_shared): the up-looking recurrence of QDLDL with the updates
-// of each reach entry spread over the lanes.
+// of each reach entry spread over the lanes.  The kernel stages all of them in
+// LDS; patterns whose tables do not fit (shared_factor_lds above the LDS of a
+// workgroup) use the one-lane row-program kernel instead.
 struct SharedFactorProg {
-  int32_t n = 0, nnz_l = 0, nsteps = 0;
-  const int32_t* row_ptr = nullptr;   // [n+1] reach entries of row k: [row_ptr[k], row_ptr[k+1])
-  const int4* steps = nullptr;        // [nsteps] {c, L slot of L[k,c], entries of column c before row k, lp[c]}
-  const int32_t* init_ptr = nullptr;  // [n+1]
-  const int2* init = nullptr;         // {row, original KKT slot}: upper entries of column k
-  const int32_t* diag = nullptr;      // [n] original KKT slot of K[k,k]
-  const int32_t* li = nullptr;        // [nnz_l] L row indices
+  int32_t n = 0, nnz_l = 0, nsteps = 0, nnz_kkt = 0, ninit = 0;
+  // row k: {end of its reach steps, first init entry, end of init entries,
+  // original KKT slot of K[k,k]}
+  const int4* rows = nullptr;      // [n]
+  // reach step: {c | (entries of column c before row k) << 16, first L slot of column c}
+  const int2* steps = nullptr;     // [nsteps]
+  const int32_t* li = nullptr;     // [nnz_l] L row indices
+  const int2* init = nullptr;      // [ninit] {row, original KKT slot}: upper entries of column k
 };
 
-// LDS bytes of k_osqp_factor_shared: row workspace, D⁻¹, L values and rows.
+// LDS layout of k_osqp_factor_shared (float offsets): rows (4n), init entries
+// (2 ninit), steps (2 nsteps), row workspace Y (n + 1: a dummy row for idle
+// lanes), D⁻¹ (n), L values (nnz_l + 1), L rows (nnz_l + 1), assembled KKT
+// values (nnz_kkt).  The 16- and 8-byte records come first, so they stay
+// aligned.
+__host__ __device__ inline int64_t shared_factor_lds_floats(const SharedFactorProg& F) {
+  return 4 * (int64_t)F.n + 2 * (int64_t)F.ninit + 2 * (int64_t)F.nsteps + 2 * (int64_t)F.n + 1 +
+         2 * ((int64_t)F.nnz_l + 1) + (int64_t)F.nnz_kkt;
+}
 inline size_t shared_factor_lds(const SharedFactorProg& F) {
-  return (size_t)(2 * (int64_t)F.n + 2 * (int64_t)F.nnz_l) * sizeof(float);
+  return (size_t)shared_factor_lds_floats(F) * sizeof(float);
 }
 
 }  // namespace wmpc
