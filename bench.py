@@ -217,7 +217,9 @@ def _osqp_leg(args, per_instance: bool, stream):
     ref = orc.solve(p32, a32, q[cols].astype(np.float32), l[cols].astype(np.float32), u[cols].astype(np.float32))
     got = [v[:, cols].t().cpu().numpy() for v in (x, z, y)]
     errs = {k: _rel_err(g, r) for k, g, r in zip(("x", "z", "y"), got, ref[:3])}
-    return dict(plan=plan, wall=wall, dev_ms=dev_ms, value=b * args.steps / wall, errs=_err_summary(errs))
+    kernel = f"wmpc_admm_shared_{dev.pattern_key():016x}" if dev.specialized and not per_instance else None
+    return dict(plan=plan, wall=wall, dev_ms=dev_ms, value=b * args.steps / wall, errs=_err_summary(errs),
+                kernel=kernel)
 
 
 # --------------------------------------------------------------------------- fwd + adjoint (cart-pole)
@@ -378,7 +380,9 @@ def _hetero_leg(args, stream):
                       "springs/Q/R (seed 2), shared P/A per pattern, factor per instance, 25 iterations, warm "
                       "start = previous solution shifted one stage (gather launches inside the step)",
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_step": u4, "kernel": "k_osqp_hetero"},
+                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_step": u4, "kernel": "k_osqp_hetero",
+                         "traffic": _profile_value("osqp_hetero_nine_patterns_b200k_pmc.txt",
+                                                   "traffic_bytes_per_launch")},
             "err_vs_fp64": _err_summary(errs)}
 
 
@@ -587,6 +591,7 @@ def main() -> int:
         u2 = _u2_bytes(plan, 25)
         ach = u2 * b / (r["dev_ms"] * 1e-3) / 1e9
         traffic = _profile_value("osqp_admm_chain_n10_b65536_pmc.txt", "traffic_bytes_per_launch")
+        kname = "k_osqp_factor_shared + " + (r["kernel"] or "k_osqp_admm<true, false>")
         line.update({
             "value": r["value"], "unit": "QP solves/s", "ms_per_step": r["wall"] * 1e3 / args.steps,
             "config": {"workload": f"linear-MPC QP, mass chain nx=12 nu=4 N=10 (n={plan.n}, m={plan.m}, "
@@ -595,11 +600,13 @@ def main() -> int:
                        "batch": b, "max_iter": 25, "parallelism": "single GPU, one lane per instance"},
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "model_bytes_per_qp": u2,
-                         "kernel": "k_osqp_factor + k_osqp_admm", "kernel_ms": r["dev_ms"],
-                         "note": "model U2 counts L and Dinv streamed per instance; with P/A shared by the "
-                                 "batch the factor is one wave-uniform stream read through the scalar cache, "
-                                 "so the model overstates HBM traffic (traffic = measured bytes per launch); "
-                                 "the kernel is instruction-issue/latency bound (one wave per SIMD)"},
+                         "kernel": kname, "kernel_ms": r["dev_ms"],
+                         "measured_GBps": None if traffic is None else traffic / (r["dev_ms"] * 1e-3) / 1e9,
+                         "note": "achieved = U2 model bytes (L and Dinv streamed per instance) / kernel time; "
+                                 "with P/A shared by the batch the factor is one wave-uniform, cache-resident "
+                                 "stream, so HBM sees only the per-instance iterate (traffic = measured bytes "
+                                 "per launch, measured_GBps = traffic / kernel time); the kernel is bound by "
+                                 "operand latency at one wavefront per SIMD"},
             "err_vs_fp64": r["errs"],
         })
     if "adjoint" in legs:
@@ -636,7 +643,10 @@ def main() -> int:
             "value": r["value"], "unit": "QP solves/s", "batch": b, "kernel_ms": r["dev_ms"],
             "config": "headline QP with per-instance P (x U[0.5,2]) and A: factor per instance",
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_qp": u2},
+                         "frac": ach / HBM_PEAK_GBS, "model_bytes_per_qp": u2,
+                         "kernel": "k_osqp_admm<false, true>", "kernel_ms": r["dev_ms"],
+                         "traffic": _profile_value("osqp_admm_per_instance_chain_n10_b65536_pmc.txt",
+                                                   "traffic_bytes_per_launch")},
             "err_vs_fp64": r["errs"],
         }
     if "cpu" in legs:
