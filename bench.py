@@ -14,8 +14,13 @@ library's batch-major layout.
 Extra legs on the same JSON line:
 * ``fwd_adjoint``: forward + adjoint QP solves/s on the cart-pole MPC
   (N = 30, |u| <= 10, Q = diag(exp theta_q), R = exp theta_r per instance,
-  batch 16,384): the forward solve, the loss cotangent, and one fused adjoint
-  launch with the reference's 100 refinement steps;
+  batch 16,384): the forward solve, the loss cotangent, and one fused launch
+  of the KKT-form adjoint (the limit of the reference's refinement: one
+  factorization on the forward KKT pattern, 3 refinement steps), with its
+  error against the FP64 limit;
+* ``fwd_adjoint_reference_algorithm``: the same with the reference's own
+  adjoint (2*half-square matrix, 100 refinement steps) and its error against
+  the FP64 run of the same algorithm;
 * ``hetero``: 200,000 instances over the nine mass-chain patterns
   ((nx, nu) in {(4,2), (12,4), (24,8)} x N in {10, 20, 40}) in one
   persistent launch, warm-started from the previous solution shifted one
@@ -58,7 +63,8 @@ def _parse():
     ap.add_argument("--legs", default="osqp,adjoint,hetero,sqp,qdldl,per_instance,cpu",
                     help="comma-separated subset of osqp,adjoint,hetero,sqp,qdldl,per_instance,cpu")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--refine", type=int, default=100, help="adjoint refinement steps")
+    ap.add_argument("--refine", type=int, default=100, help="refinement steps of the reference adjoint")
+    ap.add_argument("--kkt-refine", type=int, default=3, help="refinement steps of the KKT-form adjoint")
     ap.add_argument("--ipw", type=int, default=0,
                     help="instances per wavefront of every kernel (0 = the library's automatic choice)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -225,7 +231,7 @@ def _osqp_leg(args, per_instance: bool, stream):
 # --------------------------------------------------------------------------- fwd + adjoint (cart-pole)
 
 
-def _adjoint_leg(args, stream):
+def _adjoint_leg(args, stream, mode="reference"):
     import numpy as np
     import torch
 
@@ -234,17 +240,19 @@ def _adjoint_leg(args, stream):
     from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
     from warpmpc_amd.problems import OSQP_INFTY, cartpole_dynamics, mpc_qp
 
-    b, n_h, refine = 16384, 30, args.refine
+    b, n_h = 16384, 30
+    refine = args.refine if mode == "reference" else args.kkt_refine
     ad, bd = cartpole_dynamics()
     prob = mpc_qp(ad, bd, n_h, np.ones(4), np.ones(1), x_bound=OSQP_INFTY, u_bound=10.0)
     s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
     plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s, derivatives=True)
-    solver = compile_osqp(plan, "float32", derivatives=True, derivative_refinement_iters=refine)
+    solver = compile_osqp(plan, "float32", derivatives=True, derivative_refinement_iters=refine,
+                          derivative_mode=mode, derivative_kkt_refinement_iters=refine)
     dev, adev = solver.extras["device_plan"], solver.extras["adjoint_plan"]
     lib = _lib.load()
     dp = plan.derivative_plan
-    _log(f"fwd+adjoint: cart-pole N={n_h} (n={plan.n}, m={plan.m}, adjoint dim {dp.adjoint_dim}), batch {b}, "
-         f"R={refine}")
+    _log(f"fwd+adjoint ({mode} form): cart-pole N={n_h} (n={plan.n}, m={plan.m}, adjoint dim {dp.adjoint_dim}), "
+         f"batch {b}, {refine} refinement steps")
     rng = np.random.default_rng(3)
     tq = rng.normal(0.0, 0.5, size=(b, 4))
     tr = rng.normal(0.0, 0.5, size=(b, 1))
@@ -299,27 +307,36 @@ def _adjoint_leg(args, stream):
     steps = max(2, args.steps // 2)
     _, fwd_ms = _timed(forward, steps, 1, stream)
     _, adj_ms = _timed(adjoint, steps, 1, stream)
-    # the same adjoint in the reference's algorithm by the oracle, FP64 and FP32
     cols = np.random.default_rng(1).choice(b, size=96, replace=False)
     f64 = lambda v: v[:, cols].t().cpu().numpy().astype(np.float64)  # noqa: E731
     args_o = (np.asarray(pv[cols], np.float32).astype(np.float64), plan.a_matrix.data[None],
               np.asarray(l[cols], np.float32).astype(np.float64), np.asarray(u[cols], np.float32).astype(np.float64),
               f64(x), f64(y), f64(xbar), f64(ybar))
-    adj = oracle.OracleAdjoint(plan, refine=refine)
-    r64 = adj.compute(*args_o)[3:]
-    r32 = adj.compute(*args_o, dtype=np.float32)[3:]
     got = [f64(v) for v in (dq, dl, du, dpm, dam)]
     names = ("dq", "dl", "du", "dp", "da")
-    errs = {k: _rel_err(g, r) for k, g, r in zip(names, got, r64)}
-    spread = {k: _rel_err(a, r) for k, a, r in zip(names, r32, r64)}
-    # U3: B_qp + s [n + 2 (nnz_P + nnz_A) + (n + 2m) + R (2 nnz_L' + n_K') + F (nnz_K' + nnz_L' + n_K')]
-    q2 = dp.qdldl_plan
     u2 = _u2_bytes(plan, 25)
+    out = dict(b=b, wall=wall, dev_ms=dev_ms, fwd_ms=fwd_ms, adj_ms=adj_ms, steps=steps, u2=u2, plan=plan, dp=dp,
+               refine=refine, mode=mode)
+    if mode == "reference":
+        # the same adjoint in the reference's algorithm by the oracle, FP64 and FP32
+        adj = oracle.OracleAdjoint(plan, refine=refine)
+        r64 = adj.compute(*args_o)[3:]
+        r32 = adj.compute(*args_o, dtype=np.float32)[3:]
+        out["errs"] = _err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, r64)})
+        out["spread"] = _err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r32, r64)})
+        # U3: B_qp + s [n + 2 (nnz_P + nnz_A) + (n + 2m) + R (2 nnz_L' + n_K') + F (nnz_K' + nnz_L' + n_K')]
+        q2 = dp.qdldl_plan
+    else:
+        # the limit of the reference's refinement, FP64 (dense solve of M0)
+        ex = oracle.adjoint_exact(plan, *args_o)[3:]
+        out["errs"] = _err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, ex)})
+        r101 = oracle.OracleAdjoint(plan, refine=args.refine).compute(*args_o)[3:]
+        out["reference_algorithm_vs_limit"] = _err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r101, ex)})
+        q2 = plan.qdldl_plan  # the forward KKT pattern
     adj_bytes = S * (n + 2 * (plan.nnz_p + plan.nnz_a) + (n + 2 * m) + (refine + 1) * (2 * q2.nnz_l + q2.n)
                      + 1 * (q2.nnz_a + q2.nnz_l + q2.n))
-    return dict(b=b, wall=wall, dev_ms=dev_ms, fwd_ms=fwd_ms, adj_ms=adj_ms, steps=steps, u3=u2 + adj_bytes,
-                adj_bytes=adj_bytes, u2=u2,
-                plan=plan, dp=dp, errs=_err_summary(errs), spread=_err_summary(spread), refine=refine)
+    out.update(u3=u2 + adj_bytes, adj_bytes=adj_bytes, q2=q2)
+    return out
 
 
 # --------------------------------------------------------------------------- heterogeneous batch (C2)
@@ -609,26 +626,41 @@ def main() -> int:
                                  "operand latency at one wavefront per SIMD"},
             "err_vs_fp64": r["errs"],
         })
-    if "adjoint" in legs:
-        r = _adjoint_leg(args, stream)
+    for leg, mode in (("adjoint", "kkt"), ("adjoint_reference", "reference")):
+        if leg not in legs and not (leg == "adjoint_reference" and "adjoint" in legs):
+            continue
+        r = _adjoint_leg(args, stream, mode)
         # dominant kernel: the adjoint launch, with the adjoint part of U3
         ach = r["adj_bytes"] * r["b"] / (r["adj_ms"] * 1e-3) / 1e9
-        q2 = r["dp"].qdldl_plan
-        line["fwd_adjoint"] = {
+        q2 = r["q2"]
+        kkt = mode == "kkt"
+        prof = "osqp_adjoint_kkt_cartpole_n30_b16384_pmc.txt" if kkt else "osqp_adjoint_cartpole_n30_b16384_pmc.txt"
+        entry = {
             "value": r["b"] * r["steps"] / r["wall"], "unit": "fwd+adjoint QP solves/s", "batch": r["b"],
             "ms_per_step": r["wall"] * 1e3 / r["steps"], "forward_ms": r["fwd_ms"], "adjoint_ms": r["adj_ms"],
+            "fwd_only_qp_per_s": r["b"] / (r["fwd_ms"] * 1e-3),
             "R": r["refine"] + 1, "F": 1,
-            "config": f"cart-pole N=30 (n={r['plan'].n}, m={r['plan'].m}), per-instance Q/R, 25 ADMM iterations; "
-                      f"adjoint of dim {q2.n} (nnz {q2.nnz_a}, nnz_L {q2.nnz_l}) factored per instance, "
-                      f"{r['refine']} refinement steps",
+            "config": (f"cart-pole N=30 (n={r['plan'].n}, m={r['plan'].m}), per-instance Q/R, 25 ADMM iterations; "
+                       + (f"KKT-form adjoint: one factorization of the forward KKT pattern (dim {q2.n}, nnz_L "
+                          f"{q2.nnz_l}) and {r['refine']} refinement steps against the exact system"
+                          if kkt else
+                          f"reference adjoint: matrix of dim {q2.n} (nnz {q2.nnz_a}, nnz_L {q2.nnz_l}) factored "
+                          f"per instance, {r['refine']} refinement steps")),
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "kernel": "k_osqp_adjoint", "kernel_ms": r["adj_ms"],
-                         "model_bytes_per_instance": r["adj_bytes"], "u3_bytes_per_qp": r["u3"],
-                         "traffic": _profile_value("osqp_adjoint_cartpole_n30_b16384_pmc.txt",
-                                                   "traffic_bytes_per_launch")},
+                         "frac": ach / HBM_PEAK_GBS, "kernel": "k_osqp_adjoint_kkt" if kkt else "k_osqp_adjoint",
+                         "kernel_ms": r["adj_ms"], "model_bytes_per_instance": r["adj_bytes"],
+                         "u3_bytes_per_qp": r["u3"],
+                         "traffic": _profile_value(prof, "traffic_bytes_per_launch")},
             "err_vs_fp64": r["errs"],
-            "reference_fp32_spread": r["spread"],
         }
+        if kkt:
+            entry["err_reference"] = "FP64 limit of the reference's refinement (M0 solved exactly)"
+            entry["reference_algorithm_vs_limit"] = r["reference_algorithm_vs_limit"]
+            line["fwd_adjoint"] = entry
+        else:
+            entry["err_reference"] = "FP64 run of the reference's algorithm with the same R"
+            entry["reference_fp32_spread"] = r["spread"]
+            line["fwd_adjoint_reference_algorithm"] = entry
     if "hetero" in legs:
         line["hetero"] = _hetero_leg(args, stream)
     if "sqp" in legs:

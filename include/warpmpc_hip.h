@@ -336,17 +336,58 @@ typedef struct wmpc_adjoint_plan wmpc_adjoint_plan;
 int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* desc, const wmpc_qdldl_plan* adj,
                              wmpc_adjoint_plan** out);
 int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan);
+
+/*
+ * KKT form of the same adjoint: the limit its refinement converges to,
+ * computed on the forward KKT pattern.  No counterpart in the reference,
+ * whose adjoint (solver.py:1047-1139) refines a solve of the eps-perturbed
+ * 2*half-square matrix M toward M0 sol = rhs, and converges at a rate of
+ * eps / (sigma_min(U)^2 + eps) per step; with U nonsingular that limit is
+ * sol = [0; b], U b = r.  Substituting c = +-lambda * b_lambda on rows with a
+ * positive multiplier (b_nu on equality rows) turns U b = r into a symmetric
+ * quasi-definite system with the forward KKT's pattern,
+ *   [[P, Aᵀ], [A, diag(d)]], d = s/lambda | 0 (equality) | -1e30 (decoupled),
+ * which wmpc_osqp_adjoint_f32 factors once (sigma on P's diagonal, min(d,0) -
+ * delta on the constraint diagonal) and refines `refine` times against the
+ * exact matrix.  Outputs as the reference's mapping of sol
+ * (solver.py:1141-1192).  `kkt` is the QDLDL plan of the forward KKT pattern
+ * (FixedOSQPPlan.qdldl_plan), referenced, not copied.  Host pointers, read
+ * during creation.
+ */
+typedef struct wmpc_adjoint_kkt_desc {
+  int32_t n, m, nnz_p, nnz_a, nnz_kkt;
+  int32_t n_l, n_u, n_eq;
+  double delta;                   /* regularisation of the constraint diagonal */
+  const int32_t* kkt_rows;        /* [nnz_kkt] original upper-CSC row of each KKT slot */
+  const int32_t* kkt_cols;        /* [nnz_kkt] its column */
+  const int32_t* p_to_kkt;        /* [nnz_p] */
+  const double* p_diag_sigma;     /* [nnz_p] sigma on diagonal entries, else 0 */
+  const int32_t* a_to_kkt;        /* [nnz_a] */
+  const int32_t* lower_indices;   /* [n_l] */
+  const int32_t* upper_indices;   /* [n_u] */
+  const int32_t* eq_indices;      /* [n_eq] */
+  const int32_t* p_upper_rows;    /* [nnz_p] */
+  const int32_t* p_upper_cols;    /* [nnz_p] */
+  const int32_t* a_rows;          /* [nnz_a] */
+  const int32_t* a_cols;          /* [nnz_a] */
+} wmpc_adjoint_kkt_desc;
+int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* desc, const wmpc_qdldl_plan* kkt,
+                                 wmpc_adjoint_plan** out);
+/* 1 for a plan from wmpc_adjoint_kkt_plan_create, 0 otherwise, -1 on NULL. */
+int32_t wmpc_adjoint_is_kkt_form(const wmpc_adjoint_plan* plan);
 /* Adjoint dimension 2*half (the width of `sol`). */
 int32_t wmpc_adjoint_dim(const wmpc_adjoint_plan* plan);
 /* Floats of scratch wmpc_osqp_adjoint_f32 needs at instance stride ld:
- * (nnz_l + 4*dim + 2*(n_l + n_u)) * ld.  -1 on bad input. */
+ * (nnz_l + 4*dim + 2*(n_l + n_u)) * ld, or for the KKT form
+ * (nnz_l of the KKT + 4*(n + m) + 3*m) * ld.  -1 on bad input. */
 int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld);
 
 /*
  * adjoint_derivative_compute + get_vec + get_mat (solver.py:1047-1192) in one
  * launch, one lane per instance: multipliers and slacks from (x, y), the
  * eps-regularised adjoint matrix assembled and factored, the solve, `refine`
- * refinement steps against the unperturbed matrix, then
+ * refinement steps against the unperturbed matrix (for a KKT-form plan: the
+ * system described at wmpc_adjoint_kkt_plan_create), then
  *   dq = rx [n][ld], dl = ryl, du = -ryu [m][ld],
  *   dp [nnz_p][ld] (may be NULL), da [nnz_a][ld] (may be NULL),
  *   sol [dim][ld] (may be NULL), the refined adjoint solution.

@@ -348,6 +348,80 @@ class OracleAdjoint:
         return tuple(out)
 
 
+def adjoint_exact(plan, p_values, a_values, l, u, x, y, dx, dy):
+    """The reference adjoint (``solver.py:1047-1192``) with its refinement
+    carried to convergence, in float64 numpy.
+
+    The reference solves M0 sol = rhs, M0 = [[I, U], [Uᵀ, 0]] with
+    U = [[P, Gᵀ diag(lambda), Aeqᵀ], [G, diag(s), 0], [Aeq, 0, 0]] (G, h, lambda,
+    s as in the reference), iteratively refining a solve with the
+    eps-perturbed matrix.  The limit of that refinement is sol = [0; U⁻¹ r]
+    (a + U b = r, Uᵀ a = 0), which this function forms directly: U is built
+    dense per instance and solved with ``numpy.linalg.solve``.  The outputs
+    follow the reference's mapping (``solver.py:1141-1192``).  Returns
+    ``(sol, ryl, ryu, dq, dl, du, dp, da)`` per instance like
+    ``OracleAdjoint.compute``.
+    """
+    dp_ = plan.derivative_plan
+    if dp_ is None:
+        raise ValueError("plan has no derivative plan (build_osqp_plan(..., derivatives=True))")
+    n, m = plan.n, plan.m
+    f = lambda v, w: np.atleast_2d(np.asarray(v, dtype=np.float64))  # noqa: E731
+    pv, av = f(p_values, plan.nnz_p), f(a_values, plan.nnz_a)
+    ins = [f(v, 0) for v in (l, u, x, y, dx, dy)]
+    b = max(pv.shape[0], av.shape[0], *(v.shape[0] for v in ins))
+    pv = np.broadcast_to(pv, (b, plan.nnz_p))
+    av = np.broadcast_to(av, (b, plan.nnz_a))
+    l, u, x, y, dx, dy = (np.broadcast_to(v, (b, w)) for v, w in zip(ins, (m, m, n, m, n, m)))
+    lo, up, eq = (np.asarray(i, dtype=np.int64) for i in (dp_.lower_indices, dp_.upper_indices, dp_.eq_indices))
+    nl, nu, ne = lo.size, up.size, eq.size
+    ni = nl + nu
+    half = n + ni + ne
+    pr, pc = (np.asarray(i, dtype=np.int64) for i in (dp_.p_upper_rows, dp_.p_upper_cols))
+    ar, ac = np.asarray(plan.a_rows, dtype=np.int64), np.asarray(plan.a_cols, dtype=np.int64)
+    outs = [np.zeros((b, w)) for w in (2 * half, m, m, n, m, m, plan.nnz_p, plan.nnz_a)]
+    for k in range(b):
+        pm = np.zeros((n, n))
+        pm[pr, pc] = pv[k]
+        pm[pc, pr] = pv[k]
+        am = np.zeros((m, n))
+        am[ar, ac] = av[k]
+        g = np.vstack([-am[lo], am[up]])
+        h = np.concatenate([-l[k, lo], u[k, up]])
+        y_u = np.maximum(y[k], 0.0)
+        y_l = -np.minimum(y[k], 0.0)
+        lam = np.concatenate([y_l[lo], y_u[up]])
+        s = g @ x[k] - h
+        um = np.zeros((half, half))
+        um[:n, :n] = pm
+        um[:n, n:n + ni] = (lam[:, None] * g).T
+        um[:n, n + ni:] = am[eq].T
+        um[n:n + ni, :n] = g
+        um[n:n + ni, n:n + ni] = np.diag(s)
+        um[n + ni:, :n] = am[eq]
+        yeq = y[k, eq]
+        d_nu = np.where(yeq >= 0.0, dy[k, eq], -dy[k, eq])
+        r = np.concatenate([-dx[k], -dy[k, lo], -dy[k, up], -d_nu])
+        bsol = np.linalg.solve(um, r)
+        sol = np.concatenate([np.zeros(half), bsol])
+        rx = bsol[:n]
+        r_yl = np.zeros(m)
+        r_yu = np.zeros(m)
+        r_yl[lo] = -bsol[n:n + nl]
+        r_yu[up] = bsol[n + nl:n + ni]
+        es = bsol[n + ni:]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r_yu[eq] = np.where(yeq >= 0.0, es / yeq, 0.0)
+            r_yl[eq] = np.where(yeq >= 0.0, 0.0, -es / yeq)
+        ryl = -(r_yl * y_l)
+        ryu = r_yu * y_u
+        dpv = 0.5 * (rx[pr] * x[k, pc] + rx[pc] * x[k, pr])
+        dav = (y_u[ar] - y_l[ar]) * rx[ac] + (ryu[ar] - ryl[ar]) * x[k, ac]
+        for o, v in zip(outs, (sol, ryl, ryu, rx, ryl, -ryu, dpv, dav)):
+            o[k] = v
+    return tuple(outs)
+
+
 def num_threads() -> int:
     return int(lib().oracle_num_threads())
 
@@ -357,5 +431,5 @@ def set_threads(n: int) -> None:
     lib().oracle_set_num_threads(int(n))
 
 
-__all__ = ["OracleAdjoint", "OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor",
+__all__ = ["OracleAdjoint", "adjoint_exact", "OracleOSQP", "OraclePattern", "build", "lib", "num_threads", "qdldl_factor",
            "qdldl_solve", "set_threads"]

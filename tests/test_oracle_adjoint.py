@@ -273,3 +273,46 @@ def test_no_derivative_plan_is_an_error():
     plan = build_osqp_plan(sp.csc_matrix(np.triu(p)), sp.csc_matrix(a), l, u, TIGHT)
     with pytest.raises(ValueError):
         oracle.OracleAdjoint(plan)
+
+
+# --------------------------------------------------------------------------- converged limit
+
+
+@pytest.mark.parametrize("name", list(CASES) + ["random_3", "random_11", "random_19"])
+def test_exact_adjoint_is_the_limit_of_the_refinement(name):
+    """adjoint_exact (dense U solve) equals the C restatement with the
+    refinement run to convergence, and the dense active-set derivative."""
+    p, a, l, u, q = _problem(name)
+    plan = _plan(np.triu(p), a, l, u)
+    orc = oracle.OracleOSQP(plan)
+    x, y, *_ = _solve(orc, plan, plan.p_upper.data[None], plan.a_matrix.data[None], q[None], l[None], u[None])
+    x, y = x[0], y[0]
+    rng = np.random.default_rng(2)
+    xb, yb = rng.standard_normal(plan.n), rng.standard_normal(plan.m)
+    ex = oracle.adjoint_exact(plan, plan.p_upper.data, plan.a_matrix.data, l, u, x, y, xb, yb)
+    ref = _adjoint(plan, x, y, xb, yb, l, u, refine=400)
+    for key, v in zip(("sol", "ryl", "ryu", "q", "l", "u", "p", "a"), ex):
+        np.testing.assert_allclose(v[0], ref[key], atol=1e-9, rtol=1e-9, err_msg=key)
+    dense = _dense_active_set(plan, q, l, u, x, y, xb, yb)
+    for key, v in zip(("q", "l", "u", "p", "a"), ex[3:]):
+        np.testing.assert_allclose(v[0], dense[key], atol=1e-9, err_msg=key)
+
+
+def test_exact_adjoint_differs_from_a_short_refinement_on_an_ill_conditioned_qp():
+    """With cond(U) large the reference's 100 refinement steps stop short of
+    the limit; adjoint_exact is the limit (pinned above), so the two differ."""
+    n = 4
+    p = np.diag([1.0, 1e-3, 1.0, 1.0])
+    a = np.array([[1.0, 1.0, 0.0, 0.0], [1.0, 1.0 + 1e-4, 0.0, 0.0], [0.0, 0.0, 1.0, 0.0]])
+    l = np.array([1.0, -INF, -1.0])
+    u = np.array([1.0, 1.0, 1.0])
+    q = np.array([1.0, -2.0, 0.5, 0.3])
+    plan = _plan(np.triu(p), a, l, u)
+    orc = oracle.OracleOSQP(plan)
+    x, y, *_ = _solve(orc, plan, plan.p_upper.data[None], plan.a_matrix.data[None], q[None], l[None], u[None])
+    xb, yb = np.ones(n), np.ones(3)
+    ex = oracle.adjoint_exact(plan, plan.p_upper.data, plan.a_matrix.data, l, u, x[0], y[0], xb, yb)
+    r100 = _adjoint(plan, x[0], y[0], xb, yb, l, u, refine=100)
+    r_long = _adjoint(plan, x[0], y[0], xb, yb, l, u, refine=20000)
+    np.testing.assert_allclose(ex[3][0], r_long["q"], rtol=1e-6, atol=1e-9)
+    assert np.abs(ex[3][0] - r100["q"]).max() > 1e3 * np.abs(ex[3][0] - r_long["q"]).max()

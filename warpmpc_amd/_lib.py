@@ -130,6 +130,34 @@ class AdjointDesc(ctypes.Structure):
     ]
 
 
+class AdjointKktDesc(ctypes.Structure):
+    """Mirror of ``wmpc_adjoint_kkt_desc``."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("nnz_p", ctypes.c_int32),
+        ("nnz_a", ctypes.c_int32),
+        ("nnz_kkt", ctypes.c_int32),
+        ("n_l", ctypes.c_int32),
+        ("n_u", ctypes.c_int32),
+        ("n_eq", ctypes.c_int32),
+        ("delta", ctypes.c_double),
+        ("kkt_rows", _i32p),
+        ("kkt_cols", _i32p),
+        ("p_to_kkt", _i32p),
+        ("p_diag_sigma", _f64p),
+        ("a_to_kkt", _i32p),
+        ("lower_indices", _i32p),
+        ("upper_indices", _i32p),
+        ("eq_indices", _i32p),
+        ("p_upper_rows", _i32p),
+        ("p_upper_cols", _i32p),
+        ("a_rows", _i32p),
+        ("a_cols", _i32p),
+    ]
+
+
 class HeteroIO(ctypes.Structure):
     """Mirror of ``wmpc_osqp_hetero_io``."""
 
@@ -179,6 +207,8 @@ SIGNATURES: dict[str, tuple] = {
         _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i64, _vp]),
     "wmpc_adjoint_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointDesc), _vp, ctypes.POINTER(_vp)]),
     "wmpc_adjoint_plan_destroy": (ctypes.c_int, [_vp]),
+    "wmpc_adjoint_kkt_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointKktDesc), _vp, ctypes.POINTER(_vp)]),
+    "wmpc_adjoint_is_kkt_form": (ctypes.c_int32, [_vp]),
     "wmpc_adjoint_dim": (ctypes.c_int32, [_vp]),
     "wmpc_adjoint_work_floats": (ctypes.c_int64, [_vp, _i64]),
     "wmpc_osqp_adjoint_f32": (ctypes.c_int, [
@@ -249,5 +279,5 @@ def f64p(a: np.ndarray):
     return a.ctypes.data_as(_f64p)
 
 
-__all__ = ["AdjointDesc", "HeteroIO", "HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
+__all__ = ["AdjointDesc", "AdjointKktDesc", "HeteroIO", "HipError", "LIB_PATH", "OsqpDesc", "PROGRAM_TABLES", "QdldlPattern", "SIGNATURES", "check",
            "load", "plan_program", "plan_slots"]

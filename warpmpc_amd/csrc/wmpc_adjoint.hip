@@ -53,9 +53,28 @@ struct AdjDev {
 
 }  // namespace wmpc
 
+// Tables of the KKT-form adjoint (wmpc_adjoint_kkt_plan_create): the
+// converged limit of the refinement above, solved on the forward KKT pattern.
+namespace wmpc {
+struct AdjKktDev {
+  int32_t n = 0, m = 0, nk = 0, nl = 0, ni = 0, half = 0, nnz_p = 0, nnz_a = 0;
+  float delta = 0.f;
+  const int4* slot = nullptr;       // [nnz_kkt] {kind, index, 0, perturbed offset bits}
+  const int32_t* mv_ptr = nullptr;  // [nk + 1] rows of the full symmetric KKT
+  const int2* mv = nullptr;         // {col, slot}
+  const int32_t* arow_ptr = nullptr;  // [m + 1] rows of A
+  const int2* arow = nullptr;         // {col, A value index}
+  const int4* con = nullptr;  // [m] {lower position k or -1, upper position k or -1, eq position or -1, 0}
+  const int2* pent = nullptr;  // [nnz_p] {row, col}
+  const int2* aent = nullptr;  // [nnz_a] {row, col}
+};
+}  // namespace wmpc
+
 struct wmpc_adjoint_plan {
   wmpc::AdjDev d;
-  const wmpc_qdldl_plan* adj = nullptr;
+  wmpc::AdjKktDev k;
+  bool kkt_form = false;
+  const wmpc_qdldl_plan* adj = nullptr;  // the factored pattern: adjoint or forward KKT
   void* block = nullptr;
 };
 
@@ -206,6 +225,187 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
     for (int32_t k = 0; k < dv.nnz_a; ++k) {
       const int2 t = dv.aent[k];
       DA(k) = Y(t.x) * SOL(half + t.y) + (-DU(t.x) - DL(t.x)) * X(t.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// KKT form of the adjoint.  Its result is the limit of the refinement above:
+// with U nonsingular, M0 [a; b] = [r; 0] has a = 0 and U b = r.  Writing
+// c_i = ±lambda b_lambda for the inequality row of A row i that has a
+// positive multiplier (sign: + upper, - lower) and c_i = b_nu for an equality
+// row, U b = r becomes the symmetric system on the forward KKT pattern
+//     [[P, Aᵀ], [A, diag(d)]] [b_x; c] = [-dx; rc],
+//     d_i = s/lambda (active inequality), 0 (equality), -1e30 (no positive
+//           multiplier: the row decouples, c_i = 0),
+//     rc_i = -dy_i (upper), dy_i (lower), -d_nu_i (equality), 0.
+// The kernel factors it with sigma on P's diagonal and min(d, 0) - delta on
+// the constraint diagonal (quasi-definite), then refines against the exact
+// matrix; the outputs follow solver.py:1141-1192 with b_lambda = ±c/lambda.
+enum : int32_t { kKktP = 0, kKktA = 1, kKktDiag = 2 };
+constexpr float kDecoupled = -1e30f;
+
+template <bool PERT>
+struct AdjKktSource {
+  const int4* __restrict__ slot;
+  const float* __restrict__ p;
+  const float* __restrict__ a;
+  int64_t pes;
+  uint32_t pl;
+  CStrided dg;
+  float delta;
+  __device__ __forceinline__ float operator()(int32_t s, int64_t = 0) const {
+    const int4 t = slot[s];
+    if (t.x == kKktP) {
+      const float v = (p + (int64_t)t.y * pes)[pl];
+      return PERT ? v + __int_as_float(t.w) : v;
+    }
+    if (t.x == kKktA) return (a + (int64_t)t.y * pes)[pl];
+    const float d = dg(t.y);
+    return PERT ? fminf(d, 0.0f) - delta : d;
+  }
+};
+
+struct AdjKktRhs {
+  int32_t n;
+  CStrided dx, rc;
+  __device__ __forceinline__ float at(int32_t o) const { return o < n ? -dx(o) : rc(o - n); }
+  __device__ __forceinline__ float operator()(int32_t, int32_t o) const { return at(o); }
+};
+
+__global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
+    QdldlProg P, AdjKktDev dv, const float* __restrict__ pv, const float* __restrict__ av,
+    int64_t pm_ld, const float* __restrict__ l, const float* __restrict__ u,
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ dx,
+    const float* __restrict__ dy, float* __restrict__ lx, float* __restrict__ dinv,
+    float* __restrict__ stash, float* __restrict__ rbuf, float* __restrict__ w,
+    float* __restrict__ dg, float* __restrict__ rc, float* __restrict__ axv,
+    float* __restrict__ sol, float* __restrict__ dq, float* __restrict__ dl,
+    float* __restrict__ du, float* __restrict__ dp, float* __restrict__ da, int32_t refine,
+    int64_t batch, int64_t ld, int32_t ipw) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  int64_t b0;
+  const int64_t b = lane_instance(ipw, batch, &b0);
+  float* mywin = lds + (lane & (P.lcols - 1));
+  const int64_t pes = pm_ld ? pm_ld : 1;
+  const uint32_t pl = pm_ld ? (uint32_t)b : 0u;
+  const CStrided X{x, ld, b}, Y{y, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, DY{dy, ld, b};
+  const Strided DG{dg, ld, b}, RC{rc, ld, b}, AX{axv, ld, b}, W{w, ld, b}, RB{rbuf, ld, b};
+  // 1. per A row: A x, the coupling diagonal and the right-hand side
+  for (int32_t i = 0; i < dv.m; ++i) {
+    float ax = 0.0f;
+    for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
+      const int2 t = dv.arow[e];
+      ax += (av + (int64_t)t.y * pes)[pl] * X(t.x);
+    }
+    const int4 t = dv.con[i];
+    const float yv = Y(i), dyv = DY(i);
+    const float lu = nan_max(yv, 0.0f), ll = -nan_min(yv, 0.0f);
+    float d = kDecoupled, r = 0.0f;
+    if (t.z >= 0) {
+      d = 0.0f;
+      r = -(yv >= 0.0f ? dyv : -dyv);
+    } else if (t.y >= 0 && lu > 0.0f) {
+      d = (ax - Up(i)) / lu;
+      r = -dyv;
+    } else if (t.x >= 0 && ll > 0.0f) {
+      d = (Lo(i) - ax) / ll;
+      r = dyv;
+    }
+    if (yv != yv) d = r = yv;
+    DG(i) = d;
+    RC(i) = r;
+    AX(i) = ax;
+  }
+  const CStrided CDG{dg, ld, b};
+  const AdjKktSource<true> msrc{dv.slot, pv, av, pes, pl, CDG, dv.delta};
+  const AdjKktSource<false> m0{dv.slot, pv, av, pes, pl, CDG, dv.delta};
+  const AdjKktRhs rhs{dv.n, CStrided{dx, ld, b}, CStrided{rc, ld, b}};
+  // 2. factor with the forward substitution fused in, backward sweep
+  factor_rows<true>(P, msrc, 0, Strided{lx, ld, b}, Strided{nullptr, 0, 0}, Strided{dinv, ld, b},
+                    mywin, rhs, Strided{stash, ld, b});
+  const SweepProg SP = sweep_prog(P);
+  bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSet{W});
+  // 3. refinement against the exact matrix
+  for (int32_t it = 0; it < refine; ++it) {
+    for (int32_t i = 0; i < dv.nk; ++i) {
+      const int32_t o = P.rperm[i];
+      float s = 0.0f;
+      for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
+        const int2 t = dv.mv[e];
+        s += m0(t.y) * W(t.x);
+      }
+      RB(i) = s - rhs.at(o);
+    }
+    fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, lds, lane);
+    bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSub{W});
+  }
+  // 4. outputs; only real lanes store
+  if (b0 >= batch) return;
+  const Strided DQ{dq, ld, b}, DL{dl, ld, b}, DU{du, ld, b};
+  const int32_t n = dv.n, half = dv.half;
+  for (int32_t j = 0; j < n; ++j) DQ(j) = W(j);
+  for (int32_t i = 0; i < dv.m; ++i) {
+    const int4 t = dv.con[i];
+    const float yv = Y(i), c = W(n + i);
+    const float yu = nan_max(yv, 0.0f), yl = -nan_min(yv, 0.0f);
+    float rl = 0.0f, ru = 0.0f;
+    if (t.z >= 0) {
+      const bool pos = yv >= 0.0f;
+      ru = pos ? c / yv : 0.0f;
+      rl = pos ? 0.0f : -c / yv;
+    } else {
+      // b_lambda of a row with a positive multiplier is ±c/lambda; a row
+      // without one has lambda = 0 and contributes 0 * b_lambda
+      if (t.y >= 0 && yu > 0.0f) ru = c / yu;
+      if (t.x >= 0 && yl > 0.0f) rl = c / yl;
+    }
+    DL(i) = -(rl * yl);
+    DU(i) = -(ru * yu);
+  }
+  if (sol) {
+    // the M0 solution [a; b] = [0; b_x, b_lambda, b_nu] of the reference
+    const Strided SOL{sol, ld, b};
+    for (int32_t o = 0; o < half; ++o) SOL(o) = 0.0f;
+    for (int32_t j = 0; j < n; ++j) SOL(half + j) = W(j);
+    const int32_t base = half + n;
+    for (int32_t i = 0; i < dv.m; ++i) {
+      const int4 t = dv.con[i];
+      const float yv = Y(i), c = W(n + i), dyv = DY(i);
+      if (t.z >= 0) {
+        SOL(base + t.z) = c;
+        continue;
+      }
+      if (t.x < 0 && t.y < 0) continue;
+      float abx = 0.0f;  // A_i b_x
+      for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
+        const int2 a2 = dv.arow[e];
+        abx += (av + (int64_t)a2.y * pes)[pl] * W(a2.x);
+      }
+      const float ax = AX(i);
+      if (t.x >= 0) {  // lower row: G = -A_i, s = l - A x, rhs -dy
+        const float lam = -nan_min(yv, 0.0f);
+        SOL(base + t.x) = lam > 0.0f ? -c / lam : (-dyv + abx) / (Lo(i) - ax);
+      }
+      if (t.y >= 0) {  // upper row: G = A_i, s = A x - u, rhs -dy
+        const float lam = nan_max(yv, 0.0f);
+        SOL(base + t.y) = lam > 0.0f ? c / lam : (-dyv - abx) / (ax - Up(i));
+      }
+    }
+  }
+  if (dp) {
+    const Strided DP{dp, ld, b};
+    for (int32_t k = 0; k < dv.nnz_p; ++k) {
+      const int2 t = dv.pent[k];
+      DP(k) = 0.5f * (W(t.x) * X(t.y) + W(t.y) * X(t.x));
+    }
+  }
+  if (da) {
+    const Strided DA{da, ld, b};
+    for (int32_t k = 0; k < dv.nnz_a; ++k) {
+      const int2 t = dv.aent[k];
+      DA(k) = Y(t.x) * W(t.y) + (-DU(t.x) - DL(t.x)) * X(t.y);
     }
   }
 }
@@ -409,10 +609,171 @@ int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan) {
   return WMPC_OK;
 }
 
+int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdldl_plan* kkt,
+                                 wmpc_adjoint_plan** out) {
+  clear_error();
+  WMPC_REQUIRE(ds && kkt && out, "null argument");
+  *out = nullptr;
+  WMPC_REQUIRE(kkt->block, "host-only KKT plan: create it with wmpc_qdldl_plan_create");
+  const int32_t n = ds->n, m = ds->m, nl = ds->n_l, nu = ds->n_u, ne = ds->n_eq;
+  const int32_t nk = n + m, nz = ds->nnz_kkt;
+  WMPC_REQUIRE(n > 0 && m >= 0 && nl >= 0 && nu >= 0 && ne >= 0 && nl <= m && nu <= m && ne <= m,
+               "bad dimensions");
+  WMPC_REQUIRE(kkt->host.n == nk, "KKT plan dimension must be n + m");
+  WMPC_REQUIRE(kkt->host.nnz_a == nz, "nnz_kkt differs from the KKT plan");
+  WMPC_REQUIRE(ds->nnz_p >= 0 && ds->nnz_a >= 0 && nz >= 0, "bad counts");
+  WMPC_REQUIRE(ds->delta > 0.0, "delta must be positive");
+  WMPC_REQUIRE((nz == 0 || (ds->kkt_rows && ds->kkt_cols)) &&
+                   (ds->nnz_p == 0 || (ds->p_to_kkt && ds->p_diag_sigma && ds->p_upper_rows &&
+                                       ds->p_upper_cols)) &&
+                   (ds->nnz_a == 0 || (ds->a_to_kkt && ds->a_rows && ds->a_cols)) &&
+                   (nl == 0 || ds->lower_indices) && (nu == 0 || ds->upper_indices) &&
+                   (ne == 0 || ds->eq_indices),
+               "descriptor has a null array");
+  // value sources of the KKT slots: P, A, or the constraint diagonal
+  std::vector<int32_t> slot(4 * (size_t)nz, -1);
+  std::vector<char> set(nz, 0);
+  for (int32_t k = 0; k < ds->nnz_p; ++k) {
+    const int32_t s = ds->p_to_kkt[k];
+    WMPC_REQUIRE(s >= 0 && s < nz && !set[s], "p_to_kkt out of range or repeated");
+    WMPC_REQUIRE(ds->p_upper_rows[k] >= 0 && ds->p_upper_rows[k] < n && ds->p_upper_cols[k] >= 0 &&
+                     ds->p_upper_cols[k] < n,
+                 "p_upper_rows/cols out of range");
+    set[s] = 1;
+    slot[4 * s] = kKktP;
+    slot[4 * s + 1] = k;
+    slot[4 * s + 2] = 0;
+    slot[4 * s + 3] = float_bits(ds->p_diag_sigma[k]);
+  }
+  for (int32_t k = 0; k < ds->nnz_a; ++k) {
+    const int32_t s = ds->a_to_kkt[k];
+    WMPC_REQUIRE(s >= 0 && s < nz && !set[s], "a_to_kkt out of range or repeated");
+    set[s] = 1;
+    slot[4 * s] = kKktA;
+    slot[4 * s + 1] = k;
+    slot[4 * s + 2] = 0;
+    slot[4 * s + 3] = 0;
+  }
+  for (int32_t k = 0; k < nz; ++k) {
+    const int32_t r = ds->kkt_rows[k], c = ds->kkt_cols[k];
+    WMPC_REQUIRE(r >= 0 && c >= 0 && r <= c && c < nk, "KKT entry outside the upper triangle");
+    if (set[k]) continue;
+    WMPC_REQUIRE(r == c && r >= n, "a KKT slot outside P and A must be a constraint diagonal");
+    set[k] = 1;
+    slot[4 * k] = kKktDiag;
+    slot[4 * k + 1] = r - n;
+    slot[4 * k + 2] = 0;
+    slot[4 * k + 3] = 0;
+  }
+  // rows of the full symmetric KKT (upper entries, then mirrored ones)
+  std::vector<int32_t> mv_ptr(nk + 1, 0), mv;
+  {
+    std::vector<std::vector<int32_t>> up(nk), lo(nk);
+    for (int32_t k = 0; k < nz; ++k) {
+      up[ds->kkt_rows[k]].push_back(k);
+      if (ds->kkt_rows[k] != ds->kkt_cols[k]) lo[ds->kkt_cols[k]].push_back(k);
+    }
+    for (int32_t o = 0; o < nk; ++o) {
+      for (int32_t k : up[o]) mv.insert(mv.end(), {ds->kkt_cols[k], k});
+      for (int32_t k : lo[o]) mv.insert(mv.end(), {ds->kkt_rows[k], k});
+      mv_ptr[o + 1] = (int32_t)(mv.size() / 2);
+    }
+  }
+  // rows of A (ascending column)
+  std::vector<int32_t> arow_ptr(m + 1, 0), arow;
+  {
+    std::vector<std::vector<int32_t>> byrow(m);
+    for (int32_t k = 0; k < ds->nnz_a; ++k) {
+      WMPC_REQUIRE(ds->a_rows[k] >= 0 && ds->a_rows[k] < m && ds->a_cols[k] >= 0 && ds->a_cols[k] < n,
+                   "a_rows/a_cols out of range");
+      byrow[ds->a_rows[k]].push_back(k);
+    }
+    for (int32_t i = 0; i < m; ++i) {
+      std::stable_sort(byrow[i].begin(), byrow[i].end(),
+                       [&](int32_t a, int32_t b) { return ds->a_cols[a] < ds->a_cols[b]; });
+      for (int32_t k : byrow[i]) arow.insert(arow.end(), {ds->a_cols[k], k});
+      arow_ptr[i + 1] = (int32_t)(arow.size() / 2);
+    }
+  }
+  // per constraint row: positions of its lower / upper multiplier and of its
+  // equality multiplier, relative to the multiplier block of the M0 solution
+  std::vector<int32_t> con(4 * (size_t)m, -1);
+  for (int32_t i = 0; i < m; ++i) con[4 * i + 3] = 0;
+  for (int32_t k = 0; k < nl; ++k) {
+    const int32_t c = ds->lower_indices[k];
+    WMPC_REQUIRE(c >= 0 && c < m && con[4 * c] < 0, "lower index out of range or repeated");
+    con[4 * c] = k;
+  }
+  for (int32_t k = 0; k < nu; ++k) {
+    const int32_t c = ds->upper_indices[k];
+    WMPC_REQUIRE(c >= 0 && c < m && con[4 * c + 1] < 0, "upper index out of range or repeated");
+    con[4 * c + 1] = nl + k;
+  }
+  for (int32_t e = 0; e < ne; ++e) {
+    const int32_t c = ds->eq_indices[e];
+    WMPC_REQUIRE(c >= 0 && c < m, "eq index out of range");
+    WMPC_REQUIRE(con[4 * c] < 0 && con[4 * c + 1] < 0 && con[4 * c + 2] < 0,
+                 "an equality row is also an inequality row");
+    con[4 * c + 2] = nl + nu + e;
+  }
+  std::vector<int32_t> pent(2 * (size_t)ds->nnz_p), aent(2 * (size_t)ds->nnz_a);
+  for (int32_t k = 0; k < ds->nnz_p; ++k) {
+    pent[2 * k] = ds->p_upper_rows[k];
+    pent[2 * k + 1] = ds->p_upper_cols[k];
+  }
+  for (int32_t k = 0; k < ds->nnz_a; ++k) {
+    aent[2 * k] = ds->a_rows[k];
+    aent[2 * k + 1] = ds->a_cols[k];
+  }
+  Blob blob;
+  const size_t t_slot = blob.add(slot), t_mvp = blob.add(mv_ptr), t_mv = blob.add(mv),
+               t_arp = blob.add(arow_ptr), t_ar = blob.add(arow), t_con = blob.add(con),
+               t_pe = blob.add(pent), t_ae = blob.add(aent);
+  auto* plan = new wmpc_adjoint_plan();
+  if (int rc = blob.upload(&plan->block, "adjoint KKT plan")) {
+    delete plan;
+    return rc;
+  }
+  plan->kkt_form = true;
+  AdjKktDev& d = plan->k;
+  d.n = n;
+  d.m = m;
+  d.nk = nk;
+  d.nl = nl;
+  d.ni = nl + nu;
+  d.half = n + nl + nu + ne;
+  d.nnz_p = ds->nnz_p;
+  d.nnz_a = ds->nnz_a;
+  d.delta = (float)ds->delta;
+  d.slot = blob.at<int4>(t_slot);
+  d.mv_ptr = blob.at<int32_t>(t_mvp);
+  d.mv = blob.at<int2>(t_mv);
+  d.arow_ptr = blob.at<int32_t>(t_arp);
+  d.arow = blob.at<int2>(t_ar);
+  d.con = blob.at<int4>(t_con);
+  d.pent = blob.at<int2>(t_pe);
+  d.aent = blob.at<int2>(t_ae);
+  // the M0 width the reference's adjoint_derivative_compute returns
+  plan->d.n = n;
+  plan->d.m = m;
+  plan->d.ni = nl + nu;
+  plan->d.half = d.half;
+  plan->d.dim = 2 * d.half;
+  plan->adj = kkt;
+  *out = plan;
+  return WMPC_OK;
+}
+
 int32_t wmpc_adjoint_dim(const wmpc_adjoint_plan* plan) { return plan ? plan->d.dim : -1; }
+
+int32_t wmpc_adjoint_is_kkt_form(const wmpc_adjoint_plan* plan) {
+  return plan ? (plan->kkt_form ? 1 : 0) : -1;
+}
 
 int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld) {
   if (!plan || ld < 0) return -1;
+  if (plan->kkt_form)
+    return ((int64_t)plan->adj->host.nnz_l + 4 * (int64_t)plan->k.nk + 3 * (int64_t)plan->k.m) * ld;
   return ((int64_t)plan->adj->host.nnz_l + 4 * (int64_t)plan->d.dim + 2 * (int64_t)plan->d.ni) * ld;
 }
 
@@ -433,6 +794,32 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
   WMPC_REQUIRE(work_len >= wmpc_adjoint_work_floats(plan, ld),
                "work is smaller than wmpc_adjoint_work_floats(plan, ld)");
   const QdldlProg& P = plan->adj->prog;
+  const size_t lds = adjoint_lds(P);
+  if (lds > kMaxLdsBytes) {
+    set_error("adjoint pattern needs more LDS than a workgroup can have");
+    return WMPC_ERR_UNSUPPORTED;
+  }
+  const int32_t ipw = instances_per_wave(batch, P);
+  if (plan->kkt_form) {
+    const AdjKktDev& kv = plan->k;
+    const int64_t nk = kv.nk;
+    float* lx = work;
+    float* dinv = lx + (int64_t)P.nnz_l * ld;
+    float* stash = dinv + nk * ld;
+    float* rbuf = stash + nk * ld;
+    float* wv = rbuf + nk * ld;
+    float* dg = wv + nk * ld;
+    float* rc = dg + (int64_t)kv.m * ld;
+    float* axv = rc + (int64_t)kv.m * ld;
+    if (lds > 64 * 1024)
+      WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint_kkt),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_osqp_adjoint_kkt, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
+                       (hipStream_t)stream, P, kv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx,
+                       dinv, stash, rbuf, wv, dg, rc, axv, sol, dq, dl, du, dp, da, refine, batch, ld,
+                       ipw);
+    return check_launch("osqp adjoint (KKT form)");
+  }
   const AdjDev& dv = plan->d;
   const int64_t dim = dv.dim, nl = P.nnz_l;
   float* lx = work;
@@ -442,15 +829,9 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
   float* solw = rbuf + dim * ld;
   float* lam = solw + dim * ld;
   float* slk = lam + (int64_t)dv.ni * ld;
-  const size_t lds = adjoint_lds(P);
-  if (lds > kMaxLdsBytes) {
-    set_error("adjoint pattern needs more LDS than a workgroup can have");
-    return WMPC_ERR_UNSUPPORTED;
-  }
   if (lds > 64 * 1024)
     WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int32_t ipw = instances_per_wave(batch, P);
   hipLaunchKernelGGL(k_osqp_adjoint, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                      (hipStream_t)stream, P, dv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx,
                      dinv, stash, rbuf, sol ? sol : solw, lam, slk, dq, dl, du, dp, da, refine, batch,

@@ -39,16 +39,55 @@ class AdjointState(NamedTuple):
 
 
 class DeviceAdjointPlan:
-    """Owns the ``wmpc_adjoint_plan`` (and the QDLDL plan of the adjoint pattern)."""
+    """Owns the ``wmpc_adjoint_plan`` of a plan's derivative.
 
-    def __init__(self, plan: FixedOSQPPlan):
+    ``form="reference"``: the reference's adjoint matrix and its own QDLDL
+    plan (``wmpc_adjoint_plan_create``); ``form="kkt"``: the KKT form on the
+    forward KKT pattern (``wmpc_adjoint_kkt_plan_create``), which references
+    ``kkt_device_plan``, the forward KKT's QDLDL device plan."""
+
+    def __init__(self, plan: FixedOSQPPlan, form: str = "reference", kkt_device_plan=None,
+                 delta: float | None = None):
         dp = plan.derivative_plan
         if dp is None:
             raise ValueError("plan has no derivative plan (build_osqp_plan(..., derivatives=True))")
+        if form not in ("reference", "kkt"):
+            raise ValueError("form must be 'reference' or 'kkt'")
         lib = _lib.load()
-        self.qdldl = DevicePlan(dp.qdldl_plan)
         i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)  # noqa: E731
         f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+        self.form = form
+        handle = ctypes.c_void_p()
+        if form == "kkt":
+            if kkt_device_plan is None:
+                kkt_device_plan = DevicePlan(plan.qdldl_plan)
+            self.qdldl = kkt_device_plan
+            kkt = plan.kkt_upper
+            if not (np.array_equal(kkt.indptr, plan.qdldl_plan.a_indptr)
+                    and np.array_equal(kkt.indices, plan.qdldl_plan.a_indices)):
+                raise ValueError("plan.kkt_upper is not the pattern of plan.qdldl_plan")
+            self.keep = {
+                "kkt_rows": i32(kkt.indices),
+                "kkt_cols": i32(np.repeat(np.arange(kkt.shape[1]), np.diff(kkt.indptr))),
+                "p_to_kkt": i32(plan.p_to_kkt), "p_diag_sigma": f64(plan.p_diag_sigma),
+                "a_to_kkt": i32(plan.a_to_kkt), "lower_indices": i32(dp.lower_indices),
+                "upper_indices": i32(dp.upper_indices), "eq_indices": i32(dp.eq_indices),
+                "p_upper_rows": i32(dp.p_upper_rows), "p_upper_cols": i32(dp.p_upper_cols),
+                "a_rows": i32(plan.a_rows), "a_cols": i32(plan.a_cols),
+            }
+            d = _lib.AdjointKktDesc()
+            d.n, d.m, d.nnz_p, d.nnz_a, d.nnz_kkt = plan.n, plan.m, plan.nnz_p, plan.nnz_a, kkt.nnz
+            d.n_l, d.n_u, d.n_eq = dp.n_ineq_l, dp.n_ineq_u, dp.n_eq
+            d.delta = float(dp.eps if delta is None else delta)
+            for name, arr in self.keep.items():
+                setattr(d, name, _lib.f64p(arr) if arr.dtype == np.float64 else _lib.i32p(arr))
+            self.desc = d
+            _lib.check(lib.wmpc_adjoint_kkt_plan_create(ctypes.byref(d), self.qdldl.handle, ctypes.byref(handle)),
+                       "wmpc_adjoint_kkt_plan_create")
+            self.handle = handle
+            self.dim = int(dp.adjoint_dim)
+            return
+        self.qdldl = DevicePlan(dp.qdldl_plan)
         adj = dp.adjoint_upper
         self.keep = {
             "adj_indptr": i32(adj.indptr), "adj_indices": i32(adj.indices), "adj_data": f64(adj.data),
@@ -74,7 +113,6 @@ class DeviceAdjointPlan:
         for name, arr in self.keep.items():
             setattr(d, name, _lib.f64p(arr) if arr.dtype == np.float64 else _lib.i32p(arr))
         self.desc = d
-        handle = ctypes.c_void_p()
         _lib.check(lib.wmpc_adjoint_plan_create(ctypes.byref(d), self.qdldl.handle, ctypes.byref(handle)),
                    "wmpc_adjoint_plan_create")
         self.handle = handle
@@ -116,13 +154,15 @@ def _match_broadcast_gradient(grad, reference):
     return grad
 
 
-def attach_derivatives(compiled: dict, plan: FixedOSQPPlan, refinement_iters: int) -> dict:
-    """Adds the adjoint callables and the differentiable solves to ``compiled``."""
+def attach_derivatives(compiled: dict, plan: FixedOSQPPlan, refinement_iters: int, form: str = "reference",
+                       kkt_device_plan=None) -> dict:
+    """Adds the adjoint callables and the differentiable solves to ``compiled``
+    (``form``: the adjoint plan kind of ``DeviceAdjointPlan``)."""
     if refinement_iters < 0:
         raise ValueError("derivative_refinement_iters must be nonnegative")
     th = torch()
     lib = _lib.load()
-    dev = DeviceAdjointPlan(plan)
+    dev = DeviceAdjointPlan(plan, form=form, kkt_device_plan=kkt_device_plan)
     n, m = plan.n, plan.m
     nnz_p, nnz_a = plan.nnz_p, plan.nnz_a
     dp_ = plan.derivative_plan

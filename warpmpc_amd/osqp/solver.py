@@ -130,6 +130,8 @@ def compile_osqp(
     derivatives: bool = False,
     derivative_refinement_iters: int = 100,
     specialize: bool | str = "auto",
+    derivative_mode: str = "reference",
+    derivative_kkt_refinement_iters: int = 3,
 ) -> CompiledOSQP:
     """Compile fixed-pattern batched OSQP on the HIP backend (``solver.py:685-737``).
 
@@ -137,7 +139,15 @@ def compile_osqp(
     generated straight-line kernel (``warpmpc_amd.codegen``) run that kernel.
     ``"auto"`` attaches the code object when one was built for this pattern,
     ``True`` builds it first if needed (hipcc), ``False`` keeps the generic
-    kernel."""
+    kernel.
+
+    ``derivative_mode`` (with ``derivatives=True``): ``"reference"`` runs the
+    reference's adjoint, the eps-regularised 2*half-square adjoint matrix
+    factored per instance and ``derivative_refinement_iters`` refinement
+    solves; ``"kkt"`` computes the limit that refinement converges to, by one
+    factorization of a symmetric system with the forward KKT's pattern and
+    ``derivative_kkt_refinement_iters`` refinement solves (see
+    ``wmpc_adjoint_kkt_plan_create``)."""
     dtype = np.dtype(dtype)
     settings = plan.settings
     qd = compile_qdldl_variant(
@@ -153,6 +163,8 @@ def compile_osqp(
         level_scheduled_solve=level_scheduled_solve,
         level_scheduled_solve_threshold=level_scheduled_solve_threshold,
     )
+    if derivative_mode not in ("reference", "kkt"):
+        raise ValueError("derivative_mode must be 'reference' or 'kkt'")
     if derivatives and plan.derivative_plan is None:
         raise ValueError(
             "call build_osqp_plan(..., derivatives=True) before compile_osqp(..., derivatives=True)"
@@ -288,7 +300,11 @@ def compile_osqp(
     if derivatives:
         from .adjoint import attach_derivatives
 
-        compiled = attach_derivatives(compiled, plan, derivative_refinement_iters)
+        if derivative_mode == "kkt":
+            compiled = attach_derivatives(compiled, plan, derivative_kkt_refinement_iters, form="kkt",
+                                          kkt_device_plan=qd.extras["device_plan"])
+        else:
+            compiled = attach_derivatives(compiled, plan, derivative_refinement_iters)
     return CompiledOSQP(**compiled)
 
 
