@@ -327,8 +327,8 @@ def _adjoint_leg(args, stream, mode="reference"):
         # U3: B_qp + s [n + 2 (nnz_P + nnz_A) + (n + 2m) + R (2 nnz_L' + n_K') + F (nnz_K' + nnz_L' + n_K')]
         q2 = dp.qdldl_plan
     else:
-        # the limit of the reference's refinement, FP64 (dense solve of M0)
-        ex = oracle.adjoint_exact(plan, *args_o)[3:]
+        # the active-set limit of the reference's refinement, FP64 (dense solve of M0)
+        ex = oracle.adjoint_exact(plan, *args_o, clamp_slacks=True)[3:]
         out["errs"] = _err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, ex)})
         r101 = oracle.OracleAdjoint(plan, refine=args.refine).compute(*args_o)[3:]
         out["reference_algorithm_vs_limit"] = _err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r101, ex)})
@@ -413,22 +413,27 @@ def _sqp_leg(args, stream):
     from oracle import sqp_oracle
     from warpmpc_amd.osqp import OSQPSettings
     from warpmpc_amd.sqp import build_sparse_mpc_plan, compile_sparse_mpc_sqp
-    from warpmpc_amd.sqp.problems import quadrotor_initial_guess_and_params, quadrotor_problem
+    from warpmpc_amd.sqp.problems import QUAD_NU, QUAD_NX, quadrotor_initial_guess_and_params, quadrotor_problem
 
     b, iters = 100_000, 10
     prob = quadrotor_problem(horizon=25)
     s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, scaling=0, rho_is_vec=True)
-    plan = build_sparse_mpc_plan(prob, osqp_settings=s)
-    sqp = compile_sparse_mpc_sqp(prob, plan, osqp_settings=s)
+    plan = build_sparse_mpc_plan(prob, osqp_settings=s, derivatives=True)
+    sqp = compile_sparse_mpc_sqp(prob, plan, osqp_settings=s, derivatives=True, derivative_mode="kkt")
     op = plan.osqp_plan
     _log(f"nonlinear SQP: quadrotor N=25 (n={op.n}, m={op.m}, nnz_L={op.qdldl_plan.nnz_l}), batch {b}, "
-         f"{iters} line-search SQP iterations x 25 ADMM iterations")
+         f"{iters} line-search SQP iterations x 25 ADMM iterations, one HIP graph per iteration")
     z0, params = quadrotor_initial_guess_and_params(b, seed=4)
     z0 = torch.as_tensor(z0, dtype=torch.float32, device="cuda")
     params = torch.as_tensor(params, dtype=torch.float32, device="cuda")
+    graph = sqp.capture_step(b)
     phases = {"linearize": [], "qp": [], "line_search": []}
 
-    def solve_all(record=False):
+    def solve_graph():
+        graph.load(z0, params)
+        return graph.replay(iters)
+
+    def solve_eager(record=False):
         z, state = z0, sqp.init_state(b)
         for _ in range(iters):
             if not record:
@@ -449,8 +454,45 @@ def _sqp_leg(args, stream):
         return z
 
     solves = max(1, args.steps // 5)
-    wall, _ = _timed(lambda: solve_all(), solves, 1, stream)
-    z_fin = solve_all(record=True)
+    wall, _ = _timed(solve_graph, solves, 1, stream)
+    wall_eager, _ = _timed(solve_eager, solves, 1, stream)
+    solve_eager(record=True)
+    z_fin = solve_graph().clone()
+    same_as_eager = bool(torch.equal(z_fin, solve_eager()))
+    # dataset labels: du0/dx0 through the adjoint of the QP at the final iterate
+    # (x0 enters the first stage's rows x - x0 = 0 through l = u = x0 - x)
+    lin = sqp.linearize_nlp(z_fin, params)
+    fused = sqp.osqp.extras["adjoint_fused"]
+    warm = tuple(t.clone() for t in graph.state)
+    qp_out = {}
+
+    def labels():
+        d, _, y, prim, dual, _ = sqp.osqp.solve_warm_start(lin.p_values, lin.a_values, lin.q, lin.l, lin.u, *warm)
+        qp_out.update(d=d, y=y, prim=prim, dual=dual)
+        jac = torch.empty((b, QUAD_NU, QUAD_NX), dtype=torch.float32, device="cuda")
+        for k in range(QUAD_NU):
+            dx = torch.zeros_like(d)
+            dx[:, QUAD_NX + k] = 1.0
+            _, dl, du, *_ = fused(lin.p_values, lin.a_values, lin.l, lin.u, d, y, dx, None, want_mat=False)
+            jac[:, k] = dl[:, :QUAD_NX] + du[:, :QUAD_NX]
+        return jac
+
+    jac = labels()
+    _, label_ms = _timed(labels, max(1, args.steps // 5), 1, stream)
+    # the labels against the FP64 oracle of the same derivative (active-set KKT
+    # system at the same QP iterate) on sampled instances
+    from oracle import oracle as orc_mod
+
+    cols = np.random.default_rng(4).choice(b, size=8, replace=False)
+    f64 = lambda t: t[cols].cpu().numpy().astype(np.float64)  # noqa: E731
+    ref = np.zeros((cols.size, QUAD_NU, QUAD_NX))
+    for k in range(QUAD_NU):
+        dx = np.zeros((cols.size, op.n))
+        dx[:, QUAD_NX + k] = 1.0
+        ex = orc_mod.adjoint_exact(op, f64(lin.p_values), f64(lin.a_values), f64(lin.l), f64(lin.u), f64(qp_out["d"]),
+                                   f64(qp_out["y"]), dx, np.zeros((cols.size, op.m)), clamp_slacks=True)
+        ref[:, k] = ex[4][:, :QUAD_NX] + ex[5][:, :QUAD_NX]
+    jac_err = _rel_err(jac[cols].cpu().numpy().reshape(cols.size, -1), ref.reshape(cols.size, -1))
     # accuracy of one line-search step against the FP64 SQP oracle on a few instances
     zs, ps = z0[:4].cpu().numpy(), params[:4].cpu().numpy()
     res, _ = sqp.step(z0[:4], params[:4], state=sqp.init_state(4))
@@ -462,12 +504,20 @@ def _sqp_leg(args, stream):
     u2 = _u2_bytes(op, 25) + S * (op.n + 2 * op.m)  # warm-started QP
     ach = u2 * b / (qp_ms * 1e-3) / 1e9
     return {"value": b * solves / wall, "unit": "SQP solves/s (10 iterations each)", "qp_solves_per_s":
-            b * iters * solves / wall, "batch": b, "ms_per_solve": ms,
-            "phase_ms_per_iteration": {k: float(np.mean(v)) for k, v in phases.items()},
+            b * iters * solves / wall, "batch": b, "ms_per_solve": ms, "ms_per_solve_eager": wall_eager * 1e3 / solves,
+            "graph_equals_eager": same_as_eager,
+            "phase_ms_per_iteration_eager": {k: float(np.mean(v)) for k, v in phases.items()},
             "finite": bool(torch.isfinite(z_fin).all()),
+            "labels_du0_dx0": {"ms": label_ms, "per_s": b / (label_ms * 1e-3),
+                               "method": "QP at the final iterate (25 ADMM iterations, warm start) + 4 KKT-form "
+                                         "adjoint launches (one per input): du0/dx0 of the active-set KKT system",
+                               "err_vs_fp64": _err_summary({"du0_dx0": jac_err}),
+                               "qp_prim_res_median": float(qp_out["prim"].median()),
+                               "qp_dual_res_median": float(qp_out["dual"].median())},
             "config": "quadrotor (12 states, 4 thrusts, RK4 dt 0.02) N=25, x0 seed 4, hover initial guess, "
                       "10 SQP iterations with the filter line search, 25 ADMM iterations per QP (warm start from "
-                      "the previous QP); linearization by torch.func on the GPU, QP by the HIP OSQP",
+                      "the previous QP), each iteration one HIP graph replay; linearization by torch.func on the "
+                      "GPU, QP by the HIP OSQP",
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "kernel": "QP phase: k_osqp_admm (per-instance factor) + layout copies",
                          "model_bytes_per_qp": u2, "kernel_ms": qp_ms},

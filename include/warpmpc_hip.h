@@ -338,18 +338,22 @@ int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* desc, const wmpc_qdldl_pla
 int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan);
 
 /*
- * KKT form of the same adjoint: the limit its refinement converges to,
- * computed on the forward KKT pattern.  No counterpart in the reference,
+ * KKT form of the same adjoint: implicit differentiation of the active-set
+ * KKT system on the forward KKT pattern.  No counterpart in the reference,
  * whose adjoint (solver.py:1047-1139) refines a solve of the eps-perturbed
  * 2*half-square matrix M toward M0 sol = rhs, and converges at a rate of
  * eps / (sigma_min(U)^2 + eps) per step; with U nonsingular that limit is
  * sol = [0; b], U b = r.  Substituting c = +-lambda * b_lambda on rows with a
  * positive multiplier (b_nu on equality rows) turns U b = r into a symmetric
  * quasi-definite system with the forward KKT's pattern,
- *   [[P, Aᵀ], [A, diag(d)]], d = s/lambda | 0 (equality) | -1e30 (decoupled),
- * which wmpc_osqp_adjoint_f32 factors once (sigma on P's diagonal, min(d,0) -
- * delta on the constraint diagonal) and refines `refine` times against the
- * exact matrix.  Outputs as the reference's mapping of sol
+ *   [[P, Aᵀ], [A, diag(d)]], d = min(s, 0)/lambda | 0 (equality) | -1e30
+ *   (no positive multiplier: decoupled),
+ * which wmpc_osqp_adjoint_f32 factors once (sigma on P's diagonal, d - delta
+ * on the constraint diagonal) and refines `refine` times against the exact
+ * matrix.  Where the rows with a positive multiplier are feasible (s <= 0, a
+ * converged solution) the result is the reference's limit; at an iterate
+ * that violates such a row the slack is taken as 0, where the reference's U
+ * is nearly singular.  Outputs as the reference's mapping of sol
  * (solver.py:1141-1192).  `kkt` is the QDLDL plan of the forward KKT pattern
  * (FixedOSQPPlan.qdldl_plan), referenced, not copied.  Host pointers, read
  * during creation.

@@ -348,9 +348,14 @@ class OracleAdjoint:
         return tuple(out)
 
 
-def adjoint_exact(plan, p_values, a_values, l, u, x, y, dx, dy):
+def adjoint_exact(plan, p_values, a_values, l, u, x, y, dx, dy, clamp_slacks=False):
     """The reference adjoint (``solver.py:1047-1192``) with its refinement
     carried to convergence, in float64 numpy.
+
+    ``clamp_slacks``: the slack s of an inequality row with a positive
+    multiplier is taken as min(s, 0) (no change where those rows are feasible,
+    as at a converged solution), which makes U the Jacobian of the active-set
+    KKT system at an iterate that violates such a row.
 
     The reference solves M0 sol = rhs, M0 = [[I, U], [Uᵀ, 0]] with
     U = [[P, Gᵀ diag(lambda), Aeqᵀ], [G, diag(s), 0], [Aeq, 0, 0]] (G, h, lambda,
@@ -392,6 +397,8 @@ def adjoint_exact(plan, p_values, a_values, l, u, x, y, dx, dy):
         y_l = -np.minimum(y[k], 0.0)
         lam = np.concatenate([y_l[lo], y_u[up]])
         s = g @ x[k] - h
+        if clamp_slacks:
+            s = np.where(lam > 0.0, np.minimum(s, 0.0), s)
         um = np.zeros((half, half))
         um[:n, :n] = pm
         um[:n, n:n + ni] = (lam[:, None] * g).T

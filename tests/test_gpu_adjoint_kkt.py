@@ -1,10 +1,13 @@
 """GPU parity of the KKT form of the OSQP adjoint (derivative_mode="kkt").
 
-The KKT form computes the limit that the reference's adjoint refinement
-(warpmpc/jax_osqp/solver.py:1047-1139) converges to.  Its oracle is
-``oracle.adjoint_exact`` (the reference's formula with M0 sol = rhs solved
-exactly, densely in float64), pinned on the CPU against the C restatement run
-to convergence, the dense active-set implicit derivative and finite
+The KKT form differentiates the active-set KKT system; where the rows with a
+positive multiplier are feasible (a converged solution) that is the limit the
+reference's adjoint refinement (warpmpc/jax_osqp/solver.py:1047-1139)
+converges to.  Its oracle is ``oracle.adjoint_exact(..., clamp_slacks=True)``
+(the reference's formula with M0 sol = rhs solved exactly, densely in
+float64, slacks of rows with a positive multiplier taken as min(s, 0)),
+pinned on the CPU against the C restatement run to convergence, the dense
+active-set implicit derivative (also at infeasible iterates) and finite
 differences (tests/test_oracle_adjoint.py).
 
 * Pinned small QPs (equality rows of both multiplier signs, upper-, lower-
@@ -69,8 +72,36 @@ def test_kkt_form_matches_exact_oracle_on_pinned_cases(gpu, name):
     solver = compile_osqp(plan, np.float32, derivatives=True, derivative_mode="kkt")
     assert solver.extras["adjoint_plan"].form == "kkt"
     got = _fused(solver, pv, av, l, u, xs, ys, dx, dy, want_sol=True)
-    ref = oracle.adjoint_exact(plan, pv, av, l, u, xs, ys, dx, dy)
+    ref = oracle.adjoint_exact(plan, pv, av, l, u, xs, ys, dx, dy, clamp_slacks=True)
     for name_, g, r in zip(("dq", "dl", "du", "dp", "da", "sol"), got, (*ref[3:], ref[0])):
+        assert _err(g, r) < TOL, (name_, _err(g, r))
+
+
+@pytest.mark.parametrize("seed", [3, 11, 19])
+def test_kkt_form_on_an_infeasible_iterate(gpu, seed):
+    """(x, y) after 3 ADMM iterations: rows with a positive multiplier violate
+    their bound (slack s > 0); the KKT form takes their slack as 0 and matches
+    the active-set oracle."""
+    p, a, l, u, q = _random_qp(seed)
+    plan = build_osqp_plan(p, a, l, u, OSQPSettings(max_iter=3, rho=0.1, sigma=1e-6, alpha=1.6), derivatives=True)
+    pv, av = plan.p_upper.data[None], plan.a_matrix.data[None]
+    x, _, y, *_ = oracle.OracleOSQP(plan).solve(pv, av, q[None], l[None], u[None])
+    ax = a.toarray() @ x[0]
+    two = l < u
+    bad = two & (((y[0] > 0) & (ax > u)) | ((y[0] < 0) & (ax < l)))
+    assert bad.any()
+    rng = np.random.default_rng(seed)
+    b = 64
+    xs = np.repeat(x, b, 0).astype(np.float32).astype(np.float64)
+    ys = np.repeat(y, b, 0).astype(np.float32).astype(np.float64)
+    dx = rng.standard_normal((b, plan.n)).astype(np.float32).astype(np.float64)
+    dy = rng.standard_normal((b, plan.m)).astype(np.float32).astype(np.float64)
+    print(f"[kkt adjoint] 3-iteration iterate of random_{seed}: {int(bad.sum())} violated rows with a multiplier",
+          flush=True)
+    solver = compile_osqp(plan, np.float32, derivatives=True, derivative_mode="kkt")
+    got = _fused(solver, pv, av, l, u, xs, ys, dx, dy, want_sol=True)
+    ref = oracle.adjoint_exact(plan, pv, av, l, u, xs, ys, dx, dy, clamp_slacks=True)
+    for name_, g, r in zip(("dq", "dl", "du", "dp", "da"), got, ref[3:]):
         assert _err(g, r) < TOL, (name_, _err(g, r))
 
 
@@ -104,13 +135,13 @@ def test_kkt_form_cartpole_workload_matches_exact_oracle(gpu, u_bound):
     f64 = lambda t: t.cpu().numpy().astype(np.float64)[idx]  # noqa: E731
     args = (pv.astype(np.float32).astype(np.float64)[idx], av, l.astype(np.float32).astype(np.float64)[idx],
             u.astype(np.float32).astype(np.float64)[idx], f64(x), f64(y), f64(dx), f64(dy))
-    ref = oracle.adjoint_exact(plan, *args)[3:]
+    ref = oracle.adjoint_exact(plan, *args, clamp_slacks=True)[3:]
     r101 = oracle.OracleAdjoint(plan, refine=100).compute(*args)[3:]
     active = (np.abs(f64(y)) > 0).sum(1)
     print(f"  multipliers nonzero per instance: {active.min()}..{active.max()} (124 equality rows)", flush=True)
     for name_, g, r, rr in zip(("dq", "dl", "du", "dp", "da"), got, ref, r101):
         e, e101 = _err(g[idx], r), _err(rr, r)
-        print(f"  {name_}: KKT form vs exact {e:.2e}   reference algorithm (R=101, FP64) vs exact {e101:.2e}",
+        print(f"  {name_}: KKT form vs exact {e:.2e}   reference algorithm (R=101, FP64) vs the same limit {e101:.2e}",
               flush=True)
         assert e < TOL, (name_, e)
 

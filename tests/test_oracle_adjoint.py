@@ -316,3 +316,34 @@ def test_exact_adjoint_differs_from_a_short_refinement_on_an_ill_conditioned_qp(
     r_long = _adjoint(plan, x[0], y[0], xb, yb, l, u, refine=20000)
     np.testing.assert_allclose(ex[3][0], r_long["q"], rtol=1e-6, atol=1e-9)
     assert np.abs(ex[3][0] - r100["q"]).max() > 1e3 * np.abs(ex[3][0] - r_long["q"]).max()
+
+
+@pytest.mark.parametrize("seed", [3, 11, 19])
+def test_clamped_slacks_give_the_active_set_derivative_at_an_infeasible_iterate(seed):
+    """After 3 ADMM iterations rows with a positive multiplier violate their
+    bounds.  adjoint_exact(clamp_slacks=True) there equals the dense
+    active-set implicit derivative (active rows: y != 0), while the unclamped
+    limit differs; at a converged point clamping changes nothing."""
+    p, a, l, u, q = _problem(f"random_{seed}")
+    early = build_osqp_plan(sp.csc_matrix(np.triu(p)), sp.csc_matrix(a), l, u,
+                            OSQPSettings(max_iter=3, rho=0.1, sigma=1e-6, alpha=1.6), derivatives=True)
+    x, _, y, *_ = oracle.OracleOSQP(early).solve(early.p_upper.data[None], early.a_matrix.data[None], q[None],
+                                                 l[None], u[None])
+    x, y = x[0], y[0]
+    ax = a @ x
+    assert np.any((l < u) & (((y > 0) & (ax > u)) | ((y < 0) & (ax < l))))
+    rng = np.random.default_rng(seed)
+    xb, yb = rng.standard_normal(early.n), rng.standard_normal(early.m)
+    pv, av = early.p_upper.data, early.a_matrix.data
+    clamped = oracle.adjoint_exact(early, pv, av, l, u, x, y, xb, yb, clamp_slacks=True)
+    raw = oracle.adjoint_exact(early, pv, av, l, u, x, y, xb, yb)
+    dense = _dense_active_set(early, q, l, u, x, y, xb, yb, tol=0.0)
+    for key, v in zip(("q", "l", "u", "p", "a"), clamped[3:]):
+        np.testing.assert_allclose(v[0], dense[key], atol=1e-9, err_msg=key)
+    assert np.abs(raw[3][0] - clamped[3][0]).max() > 1e-3
+    plan = _plan(np.triu(p), a, l, u)
+    xc, _, yc, *_ = oracle.OracleOSQP(plan).solve(pv[None], av[None], q[None], l[None], u[None])
+    a1 = oracle.adjoint_exact(plan, pv, av, l, u, xc[0], yc[0], xb, yb, clamp_slacks=True)
+    a2 = oracle.adjoint_exact(plan, pv, av, l, u, xc[0], yc[0], xb, yb)
+    for v1, v2 in zip(a1, a2):
+        np.testing.assert_allclose(v1, v2, rtol=1e-9, atol=1e-12)

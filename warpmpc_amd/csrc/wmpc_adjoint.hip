@@ -230,18 +230,26 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
 }
 
 // ---------------------------------------------------------------------------
-// KKT form of the adjoint.  Its result is the limit of the refinement above:
-// with U nonsingular, M0 [a; b] = [r; 0] has a = 0 and U b = r.  Writing
-// c_i = ±lambda b_lambda for the inequality row of A row i that has a
-// positive multiplier (sign: + upper, - lower) and c_i = b_nu for an equality
-// row, U b = r becomes the symmetric system on the forward KKT pattern
+// KKT form of the adjoint: implicit differentiation of the active-set KKT
+// system.  With U nonsingular, the limit of the refinement above solves
+// M0 [a; b] = [r; 0], i.e. a = 0 and U b = r.  Writing c_i = ±lambda b_lambda
+// for the inequality row of A row i that has a positive multiplier (sign: +
+// upper, - lower) and c_i = b_nu for an equality row, U b = r becomes the
+// symmetric system on the forward KKT pattern
 //     [[P, Aᵀ], [A, diag(d)]] [b_x; c] = [-dx; rc],
-//     d_i = s/lambda (active inequality), 0 (equality), -1e30 (no positive
-//           multiplier: the row decouples, c_i = 0),
+//     d_i = min(s, 0)/lambda (inequality with a positive multiplier),
+//           0 (equality), -1e30 (no positive multiplier: the row decouples,
+//           c_i = 0),
 //     rc_i = -dy_i (upper), dy_i (lower), -d_nu_i (equality), 0.
-// The kernel factors it with sigma on P's diagonal and min(d, 0) - delta on
-// the constraint diagonal (quasi-definite), then refines against the exact
-// matrix; the outputs follow solver.py:1141-1192 with b_lambda = ±c/lambda.
+// Where the rows with a positive multiplier are feasible (s <= 0, as at a
+// converged solution) this is exactly the reference's limit.  An iterate of a
+// few ADMM iterations can violate such a row (s > 0); the slack is then taken
+// as 0, the value at the converged solution: the reference's U is nearly
+// singular at such points, the active-set system is not.  d <= 0 always, so
+// the matrix factored with sigma on P's diagonal and d - delta on the
+// constraint diagonal is quasi-definite; the kernel refines against the
+// exact matrix, and the outputs follow solver.py:1141-1192 with
+// b_lambda = ±c/lambda.
 enum : int32_t { kKktP = 0, kKktA = 1, kKktDiag = 2 };
 constexpr float kDecoupled = -1e30f;
 
@@ -262,7 +270,7 @@ struct AdjKktSource {
     }
     if (t.x == kKktA) return (a + (int64_t)t.y * pes)[pl];
     const float d = dg(t.y);
-    return PERT ? fminf(d, 0.0f) - delta : d;
+    return PERT ? d - delta : d;
   }
 };
 
@@ -307,10 +315,10 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
       d = 0.0f;
       r = -(yv >= 0.0f ? dyv : -dyv);
     } else if (t.y >= 0 && lu > 0.0f) {
-      d = (ax - Up(i)) / lu;
+      d = fminf((ax - Up(i)) / lu, 0.0f);
       r = -dyv;
     } else if (t.x >= 0 && ll > 0.0f) {
-      d = (Lo(i) - ax) / ll;
+      d = fminf((Lo(i) - ax) / ll, 0.0f);
       r = dyv;
     }
     if (yv != yv) d = r = yv;

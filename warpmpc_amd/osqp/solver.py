@@ -144,10 +144,11 @@ def compile_osqp(
     ``derivative_mode`` (with ``derivatives=True``): ``"reference"`` runs the
     reference's adjoint, the eps-regularised 2*half-square adjoint matrix
     factored per instance and ``derivative_refinement_iters`` refinement
-    solves; ``"kkt"`` computes the limit that refinement converges to, by one
-    factorization of a symmetric system with the forward KKT's pattern and
-    ``derivative_kkt_refinement_iters`` refinement solves (see
-    ``wmpc_adjoint_kkt_plan_create``)."""
+    solves; ``"kkt"`` differentiates the active-set KKT system -- the limit
+    that refinement converges to wherever the rows with a positive multiplier
+    are feasible -- by one factorization of a symmetric system with the
+    forward KKT's pattern and ``derivative_kkt_refinement_iters`` refinement
+    solves (see ``wmpc_adjoint_kkt_plan_create``)."""
     dtype = np.dtype(dtype)
     settings = plan.settings
     qd = compile_qdldl_variant(

@@ -15,6 +15,7 @@ from .sparse_mpc import (
     CompiledSparseMPCSQP,
     SparseMPCPlan,
     SparseMPCProblem,
+    SQPGraph,
     SQPLinearization,
     SQPLineSearchStepResult,
     SQPSolveResult,
@@ -32,6 +33,7 @@ __all__ = [
     "LINE_SEARCH_COST",
     "LINE_SEARCH_COST_OR_CONSTRAINT",
     "LINE_SEARCH_MAX_ITER",
+    "SQPGraph",
     "SQPLineSearchStepResult",
     "SQPLinearization",
     "SQPSolveResult",

@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 
 from .sparse_mpc import SparseMPCProblem
-from .stage import TorchStageFunction
+from .stage import TorchStageFunction, device_const
 
 
 def _torch():
@@ -29,14 +29,18 @@ def _torch():
     return torch
 
 
+_TINY_LOWER = np.array([-1.0, 0.0])
+_TINY_UPPER = np.array([1.0, 0.0])
+
+
 def tiny_problem() -> SparseMPCProblem:
     th = _torch()
 
     def first(z, zn, p):
         cost = 0.5 * ((z[..., 0] - p[..., 0]) ** 2 + 2.0 * z[..., 1] ** 2) + 0.25 * zn[..., 0] ** 2
         g = th.stack([z[..., 0] + zn[..., 0] ** 2, z[..., 1] + zn[..., 1]], dim=-1)
-        lower = th.broadcast_to(th.tensor([-1.0, 0.0], dtype=z.dtype, device=z.device), g.shape)
-        upper = th.broadcast_to(th.tensor([1.0, 0.0], dtype=z.dtype, device=z.device), g.shape)
+        lower = th.broadcast_to(device_const(_TINY_LOWER, z.dtype, z.device), g.shape)
+        upper = th.broadcast_to(device_const(_TINY_UPPER, z.dtype, z.device), g.shape)
         return cost, g, lower, upper
 
     def terminal(z, p):
@@ -98,7 +102,9 @@ def quadrotor_rk4(x, u, dt: float = QUAD_DT):
     return x + (dt / 6.0) * (k1 + 2.0 * k2 + 2.0 * k3 + k4)
 
 
-_Q = (10.0, 10.0, 10.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.1, 0.1, 0.1)
+# constants as host arrays, moved to the device once (device_const) so the
+# stage functions make no host-to-device copy per call and can be captured
+_Q = np.array([10.0, 10.0, 10.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.1, 0.1, 0.1])
 _R = 100.0
 _QN = 10.0
 
@@ -109,7 +115,7 @@ def quadrotor_problem(horizon: int = 25) -> SparseMPCProblem:
     nx, nu = QUAD_NX, QUAD_NU
 
     def tracking(x, u, target):
-        q = th.as_tensor(_Q, dtype=x.dtype, device=x.device)
+        q = device_const(_Q, x.dtype, x.device)
         e = th.cat([x[..., :3] - target, x[..., 3:]], dim=-1)
         du = u - QUAD_HOVER
         return 0.5 * (q * e * e).sum(dim=-1) + 0.5 * _R * (du * du).sum(dim=-1)

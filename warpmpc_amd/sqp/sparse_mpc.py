@@ -39,7 +39,7 @@ from .line_search import (
     filter_line_search_from_evaluations,
     make_step_lengths,
 )
-from .stage import TorchStageFunction
+from .stage import TorchStageFunction, device_const
 
 OSQP_INFTY = 1e30
 OSQP_MIN_SCALING = 1e-4
@@ -162,7 +162,52 @@ class CompiledSparseMPCSQP:
     line_search_step: Callable
     line_search_settings: FilterLineSearchSettings
     line_search_step_lengths: np.ndarray
+    capture_step: Callable | None = None
     extras: dict = field(default_factory=dict, compare=False, repr=False)
+
+
+class SQPGraph:
+    """One line-search SQP iteration for a fixed batch, captured in a HIP graph.
+
+    The graph holds the whole iteration -- linearization, the warm-started QP
+    (the fused HIP OSQP launch), the candidate evaluations and the filter line
+    search -- followed by device copies that feed its results back into its
+    inputs: the iterate ``z`` and the QP warm start.  ``replay(k)`` therefore
+    runs k SQP iterations with k graph launches and no host work in between.
+    Inputs are loaded with ``load``; ``z``, ``state`` and ``last`` (the
+    ``SQPLineSearchStepResult`` of the latest iteration) are the graph's own
+    buffers, overwritten by every replay (clone what must be kept).  No
+    counterpart in the reference, which traces the step with jax.jit
+    (sparse_mpc.py:997-1053).
+    """
+
+    def __init__(self, graph, z, params, state, last, pool=None):
+        self.graph = graph
+        self.z = z
+        self.params = params
+        self.state = state
+        self.last = last
+        self.pool = pool
+
+    @property
+    def batch_size(self) -> int:
+        return int(self.z.shape[0])
+
+    def load(self, z, params, state: OSQPWarmStart | None = None) -> None:
+        """Copies an iterate, parameters and a warm start (zeros if None) into the graph's inputs."""
+        self.z.copy_(z)
+        self.params.copy_(params)
+        for dst, src in zip(self.state, state if state is not None else (None, None, None)):
+            if src is None:
+                dst.zero_()
+            else:
+                dst.copy_(src)
+
+    def replay(self, iterations: int = 1):
+        """Runs ``iterations`` SQP iterations on the current stream; returns ``z``."""
+        for _ in range(int(iterations)):
+            self.graph.replay()
+        return self.z
 
 
 def _offsets(widths) -> np.ndarray:
@@ -364,8 +409,12 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
                            qdldl_factor_backend: str | None = None, qdldl_solve_backend: str | None = None,
                            derivatives: bool = False, derivative_refinement_iters: int = 100,
                            line_search_settings: FilterLineSearchSettings | None = None,
-                           group_repeated_stages: bool = True) -> CompiledSparseMPCSQP:
-    """Compile the SQP step functions on the HIP OSQP (sparse_mpc.py:703-1228)."""
+                           group_repeated_stages: bool = True,
+                           derivative_mode: str = "reference") -> CompiledSparseMPCSQP:
+    """Compile the SQP step functions on the HIP OSQP (sparse_mpc.py:703-1228).
+
+    ``derivative_mode`` selects the QP adjoint (``compile_osqp``): the
+    reference's algorithm, or ``"kkt"``, the limit of its refinement."""
     dtype = np.dtype(dtype)
     if dtype != np.dtype(np.float32):
         raise ValueError("the HIP backend computes in float32")
@@ -381,7 +430,7 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
                       segment_budget=segment_budget, segment_strategy=segment_strategy,
                       level_scheduled_solve=level_scheduled_solve,
                       level_scheduled_solve_threshold=level_scheduled_solve_threshold, derivatives=derivatives,
-                      derivative_refinement_iters=derivative_refinement_iters)
+                      derivative_refinement_iters=derivative_refinement_iters, derivative_mode=derivative_mode)
     th = _torch()
     groups = _groups(problem, plan, group_repeated_stages)
     n, m = plan.n_variables, plan.n_constraints
@@ -491,7 +540,7 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
         scale = ls.line_search_constraint_scale
         v0 = constraint_violation(base.constraint_l, base.g, base.constraint_u, scale)
         metric = (base.q * d).sum(dim=1)
-        steps = th.as_tensor(steps_np, device=z.device)
+        steps = device_const(steps_np, f32, z.device)
         k, b = steps.shape[0], z.shape[0]
         cand = (z[None] + steps[:, None, None] * d[None]).reshape(k * b, n)
         cost, g, lo, up = evaluate_values(cand, params.repeat(k, 1))
@@ -510,6 +559,39 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
         solve, nxt = solve_qp_data(linearize(z, params), state=state)
         return line_search_step(z, params, solve), nxt
 
+    def capture_step(batch_size: int, warmup: int = 2) -> SQPGraph:
+        """Captures one line-search iteration for ``batch_size`` instances in a
+        HIP graph (see ``SQPGraph``)."""
+        b = int(batch_size)
+        if b <= 0:
+            raise ValueError("batch_size must be positive")
+        kw = dict(dtype=f32, device="cuda")
+        zb = th.zeros((b, n), **kw)
+        pb = th.zeros((b, plan.n_parameters), **kw)
+        sb = tuple(th.zeros((b, w), **kw) for w in (n, m, m))
+
+        def body():
+            res, nxt = step(zb, pb, state=OSQPWarmStart(*sb))
+            zb.copy_(res.z_next)
+            for dst, src in zip(sb, nxt):
+                dst.copy_(src)
+            return res
+
+        # warm-up on a side stream at the default priority: index tables and
+        # constants reach the device, the caching allocator sees every size
+        side = th.cuda.Stream()
+        side.wait_stream(th.cuda.current_stream())
+        with th.cuda.stream(side):
+            for _ in range(max(1, int(warmup))):
+                body()
+        th.cuda.current_stream().wait_stream(side)
+        th.cuda.synchronize()
+        graph = th.cuda.CUDAGraph()
+        with th.cuda.graph(graph):
+            last = body()
+        th.cuda.synchronize()
+        return SQPGraph(graph, zb, pb, sb, last)
+
     def evaluate_nlp(z, params):
         return evaluate_values(as_batch(z, n, "z"), as_batch(params, plan.n_parameters, "params"))
 
@@ -522,11 +604,13 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
         compute_direction=compute_direction, fixed_step=fixed_step, step=step, line_search_step=line_search_step,
         line_search_settings=ls,
         line_search_step_lengths=steps_np,
+        capture_step=capture_step,
     )
 
 
 __all__ = [
     "CompiledSparseMPCSQP",
+    "SQPGraph",
     "SQPLineSearchStepResult",
     "SQPLinearization",
     "SQPSolveResult",

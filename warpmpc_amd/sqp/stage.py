@@ -31,6 +31,21 @@ def _torch():
     return torch
 
 
+_CONSTS: dict = {}
+
+
+def device_const(arr: np.ndarray, dtype, device):
+    """A device copy of a constant host array, made once per (array, dtype,
+    device): steps issue no host-to-device copies after their first call, so
+    they can be captured in a HIP graph."""
+    key = (id(arr), str(dtype), str(device))
+    hit = _CONSTS.get(key)
+    if hit is None or hit[0] is not arr:
+        hit = (arr, _torch().as_tensor(arr, dtype=dtype, device=device))
+        _CONSTS[key] = hit
+    return hit[1]
+
+
 @dataclass(frozen=True)
 class TorchStageFunction:
     name: str
@@ -117,10 +132,10 @@ class TorchStageFunction:
         constant = self.hess_const is not None
         grad, hess, jac = _derivatives(self.fn, self.has_next, self.z_dim, loc2, p2, hessian_too=not constant)
         dev = loc.device
-        idx = lambda a: th.as_tensor(a, dtype=th.long, device=dev)  # noqa: E731
+        idx = lambda a: device_const(a, th.long, dev)  # noqa: E731
         ge = grad[:, idx(self.grad_cols)]
         if constant:
-            he = th.as_tensor(self.hess_const, dtype=loc.dtype, device=dev).expand(loc2.shape[0], -1)
+            he = device_const(self.hess_const, loc.dtype, dev).expand(loc2.shape[0], -1)
         else:
             he = hess[:, idx(self.hess_rows), idx(self.hess_cols)]
         je = jac[:, idx(self.jac_rows), idx(self.jac_cols)]
@@ -151,4 +166,4 @@ def _derivatives(fn, has_next: bool, z_dim: int, loc, p, hessian_too: bool = Tru
     return grad, hess, jac
 
 
-__all__ = ["TorchStageFunction"]
+__all__ = ["TorchStageFunction", "device_const"]
