@@ -15,9 +15,9 @@ Extra legs on the same JSON line:
 * ``fwd_adjoint``: forward + adjoint QP solves/s on the cart-pole MPC
   (N = 30, |u| <= 10, Q = diag(exp theta_q), R = exp theta_r per instance,
   batch 16,384): the forward solve, the loss cotangent, and one fused launch
-  of the KKT-form adjoint (the limit of the reference's refinement: one
-  factorization on the forward KKT pattern, 3 refinement steps), with its
-  error against the FP64 limit;
+  of the KKT-form adjoint (the active-set limit of the reference's
+  refinement: per-instance equilibration, one factorization on the forward
+  KKT pattern, 4 refinement steps), with its error against the FP64 limit;
 * ``fwd_adjoint_reference_algorithm``: the same with the reference's own
   adjoint (2*half-square matrix, 100 refinement steps) and its error against
   the FP64 run of the same algorithm;
@@ -64,7 +64,7 @@ def _parse():
                     help="comma-separated subset of osqp,adjoint,hetero,sqp,qdldl,per_instance,cpu")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--refine", type=int, default=100, help="refinement steps of the reference adjoint")
-    ap.add_argument("--kkt-refine", type=int, default=3, help="refinement steps of the KKT-form adjoint")
+    ap.add_argument("--kkt-refine", type=int, default=4, help="refinement steps of the KKT-form adjoint")
     ap.add_argument("--ipw", type=int, default=0,
                     help="instances per wavefront of every kernel (0 = the library's automatic choice)")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -101,6 +101,16 @@ int wmpc_qdldl_plan_program(const wmpc_qdldl_plan* plan, int32_t which, int32_t*
  * value a launch of `batch` instances uses (-1 on bad input). */
 int wmpc_set_instances_per_wave(int32_t ipw);
 int32_t wmpc_instances_per_wave(int64_t batch);
+/* The same knob for the launches of one plan (every kernel that runs on its
+ * pattern: QDLDL, OSQP, adjoint): 0 = the process-wide choice above, or 8, 16,
+ * 32, 64; it takes precedence over wmpc_set_instances_per_wave.  Set by the
+ * autotuner (warpmpc_amd/autotune.py).  wmpc_qdldl_plan_instances_per_wave
+ * reports the value a launch of `batch` instances on this plan uses, capped
+ * by the plan's LDS columns (-1 on bad input); wmpc_qdldl_plan_lds_budget the
+ * plan's LDS budget in bytes (-1 on NULL).  No counterpart in the reference. */
+int wmpc_qdldl_plan_set_instances_per_wave(wmpc_qdldl_plan* plan, int32_t ipw);
+int32_t wmpc_qdldl_plan_instances_per_wave(const wmpc_qdldl_plan* plan, int64_t batch);
+int64_t wmpc_qdldl_plan_lds_budget(const wmpc_qdldl_plan* plan);
 
 /* CompiledQDLDL.factor (warp_backend.py:1302-1349; kernel :104-173):
  * a_values [nnz_a][ld] (original upper-CSC order) -> lx [nnz_l][ld],
@@ -250,7 +260,10 @@ typedef struct wmpc_osqp_hetero_io {
   float* obj;
   int64_t batch, ld;
 } wmpc_osqp_hetero_io;
-/* 1 to 12 OSQP plans (referenced, not copied; they must outlive it). */
+/* 1 to 12 OSQP plans (referenced, not copied; they must outlive it).  The
+ * patterns' programs are copied: a wmpc_qdldl_plan_set_lds_budget on one of
+ * their KKT plans afterwards makes wmpc_osqp_solve_hetero_f32 fail with
+ * WMPC_ERR_INVALID until the heterogeneous plan is created again. */
 int wmpc_osqp_hetero_plan_create(const wmpc_osqp_plan* const* plans, int32_t count,
                                  wmpc_osqp_hetero_plan** out);
 int wmpc_osqp_hetero_plan_destroy(wmpc_osqp_hetero_plan* plan);
@@ -348,9 +361,10 @@ int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan);
  * quasi-definite system with the forward KKT's pattern,
  *   [[P, Aᵀ], [A, diag(d)]], d = min(s, 0)/lambda | 0 (equality) | -1e30
  *   (no positive multiplier: decoupled),
- * which wmpc_osqp_adjoint_f32 factors once (sigma on P's diagonal, d - delta
- * on the constraint diagonal) and refines `refine` times against the exact
- * matrix.  Where the rows with a positive multiplier are feasible (s <= 0, a
+ * which wmpc_osqp_adjoint_f32 equilibrates per instance (symmetric Ruiz
+ * scaling, 2 passes), factors once (sigma on P's diagonal, -delta on the
+ * equilibrated constraint diagonal) and refines `refine` times against the
+ * exact matrix.  Where the rows with a positive multiplier are feasible (s <= 0, a
  * converged solution) the result is the reference's limit; at an iterate
  * that violates such a row the slack is taken as 0, where the reference's U
  * is nearly singular.  Outputs as the reference's mapping of sol
@@ -361,7 +375,7 @@ int wmpc_adjoint_plan_destroy(wmpc_adjoint_plan* plan);
 typedef struct wmpc_adjoint_kkt_desc {
   int32_t n, m, nnz_p, nnz_a, nnz_kkt;
   int32_t n_l, n_u, n_eq;
-  double delta;                   /* regularisation of the constraint diagonal */
+  double delta;                   /* regularisation of the equilibrated constraint diagonal */
   const int32_t* kkt_rows;        /* [nnz_kkt] original upper-CSC row of each KKT slot */
   const int32_t* kkt_cols;        /* [nnz_kkt] its column */
   const int32_t* p_to_kkt;        /* [nnz_p] */
@@ -383,7 +397,7 @@ int32_t wmpc_adjoint_is_kkt_form(const wmpc_adjoint_plan* plan);
 int32_t wmpc_adjoint_dim(const wmpc_adjoint_plan* plan);
 /* Floats of scratch wmpc_osqp_adjoint_f32 needs at instance stride ld:
  * (nnz_l + 4*dim + 2*(n_l + n_u)) * ld, or for the KKT form
- * (nnz_l of the KKT + 4*(n + m) + 3*m) * ld.  -1 on bad input. */
+ * (nnz_l of the KKT + 5*(n + m) + 3*m) * ld.  -1 on bad input. */
 int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld);
 
 /*

@@ -409,7 +409,16 @@ def adjoint_exact(plan, p_values, a_values, l, u, x, y, dx, dy, clamp_slacks=Fal
         yeq = y[k, eq]
         d_nu = np.where(yeq >= 0.0, dy[k, eq], -dy[k, eq])
         r = np.concatenate([-dx[k], -dy[k, lo], -dy[k, up], -d_nu])
-        bsol = np.linalg.solve(um, r)
+        # an inequality row without a positive multiplier has a zero column
+        # in the x rows of U: its b_lambda follows from b_x afterwards, and U
+        # is solved on the other rows (U itself is singular when such a row
+        # also has s = 0)
+        keep = np.concatenate([np.ones(n, bool), lam > 0.0, np.ones(ne, bool)])
+        bsol = np.zeros(half)
+        bsol[keep] = np.linalg.solve(um[np.ix_(keep, keep)], r[keep])
+        drop = np.flatnonzero(~keep)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            bsol[drop] = (r[drop] - um[drop, :n] @ bsol[:n]) / s[drop - n]
         sol = np.concatenate([np.zeros(half), bsol])
         rx = bsol[:n]
         r_yl = np.zeros(m)

@@ -242,7 +242,10 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
 //           c_i = 0),
 //     rc_i = -dy_i (upper), dy_i (lower), -d_nu_i (equality), 0.
 // Where the rows with a positive multiplier are feasible (s <= 0, as at a
-// converged solution) this is exactly the reference's limit.  An iterate of a
+// converged solution) this is exactly the reference's limit.  The kernel
+// equilibrates the system per instance (Ruiz, 2 passes) before factoring it
+// in float32: without that, element growth on an equality-heavy KKT (the
+// quadrotor MPC) makes the refinement diverge.  An iterate of a
 // few ADMM iterations can violate such a row (s > 0); the slack is then taken
 // as 0, the value at the converged solution: the reference's U is nearly
 // singular at such points, the active-set system is not.  d <= 0 always, so
@@ -253,33 +256,43 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
 enum : int32_t { kKktP = 0, kKktA = 1, kKktDiag = 2 };
 constexpr float kDecoupled = -1e30f;
 
-template <bool PERT>
+// Values of the KKT-form matrix.  PERT: the factored matrix (sigma on P's
+// diagonal, -delta on the constraint diagonal), else the exact one.  SCALED:
+// equilibrated, K_s = S K S with S = diag(sc) (slot.z packs the entry's row
+// and column); a decoupled row's -1e30 stays unscaled.
+template <bool PERT, bool SCALED>
 struct AdjKktSource {
   const int4* __restrict__ slot;
   const float* __restrict__ p;
   const float* __restrict__ a;
   int64_t pes;
   uint32_t pl;
-  CStrided dg;
+  CStrided dg, sc;
   float delta;
   __device__ __forceinline__ float operator()(int32_t s, int64_t = 0) const {
     const int4 t = slot[s];
+    float f = 1.0f;
+    if constexpr (SCALED) f = sc(t.z & 0xffff) * sc((uint32_t)t.z >> 16);
     if (t.x == kKktP) {
       const float v = (p + (int64_t)t.y * pes)[pl];
-      return PERT ? v + __int_as_float(t.w) : v;
+      return (PERT ? v + __int_as_float(t.w) : v) * f;
     }
-    if (t.x == kKktA) return (a + (int64_t)t.y * pes)[pl];
+    if (t.x == kKktA) return (a + (int64_t)t.y * pes)[pl] * f;
     const float d = dg(t.y);
-    return PERT ? d - delta : d;
+    if (d <= kDecoupled) return d;
+    return PERT ? d * f - delta : d * f;
   }
 };
 
 struct AdjKktRhs {
   int32_t n;
-  CStrided dx, rc;
-  __device__ __forceinline__ float at(int32_t o) const { return o < n ? -dx(o) : rc(o - n); }
+  CStrided dx, rc, sc;
+  __device__ __forceinline__ float at(int32_t o) const { return (o < n ? -dx(o) : rc(o - n)) * sc(o); }
   __device__ __forceinline__ float operator()(int32_t, int32_t o) const { return at(o); }
 };
+
+// Ruiz passes of the symmetric equilibration
+constexpr int kRuizPasses = 2;
 
 __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     QdldlProg P, AdjKktDev dv, const float* __restrict__ pv, const float* __restrict__ av,
@@ -288,7 +301,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     const float* __restrict__ dy, float* __restrict__ lx, float* __restrict__ dinv,
     float* __restrict__ stash, float* __restrict__ rbuf, float* __restrict__ w,
     float* __restrict__ dg, float* __restrict__ rc, float* __restrict__ axv,
-    float* __restrict__ sol, float* __restrict__ dq, float* __restrict__ dl,
+    float* __restrict__ scv, float* __restrict__ sol, float* __restrict__ dq, float* __restrict__ dl,
     float* __restrict__ du, float* __restrict__ dp, float* __restrict__ da, int32_t refine,
     int64_t batch, int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
@@ -326,16 +339,37 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     RC(i) = r;
     AX(i) = ax;
   }
-  const CStrided CDG{dg, ld, b};
-  const AdjKktSource<true> msrc{dv.slot, pv, av, pes, pl, CDG, dv.delta};
-  const AdjKktSource<false> m0{dv.slot, pv, av, pes, pl, CDG, dv.delta};
-  const AdjKktRhs rhs{dv.n, CStrided{dx, ld, b}, CStrided{rc, ld, b}};
-  // 2. factor with the forward substitution fused in, backward sweep
+  const CStrided CDG{dg, ld, b}, CSC{scv, ld, b};
+  const Strided SC{scv, ld, b};
+  // 2. symmetric equilibration (Ruiz): sc_o <- sc_o / sqrt(max_j |sc_o K_oj sc_j|),
+  // a decoupled row's diagonal left out; the new scales are staged in rbuf
+  {
+    const AdjKktSource<false, false> raw{dv.slot, pv, av, pes, pl, CDG, CSC, dv.delta};
+    for (int32_t o = 0; o < dv.nk; ++o) SC(o) = 1.0f;
+    for (int32_t pass = 0; pass < kRuizPasses; ++pass) {
+      for (int32_t o = 0; o < dv.nk; ++o) {
+        float mx = 0.0f;
+        for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
+          const int2 t = dv.mv[e];
+          const float v = raw(t.y);
+          if (v > kDecoupled) mx = fmaxf(mx, fabsf(v) * SC(t.x));
+        }
+        mx *= SC(o);
+        RB(o) = mx > 0.0f ? SC(o) / sqrtf(mx) : SC(o);
+      }
+      for (int32_t o = 0; o < dv.nk; ++o) SC(o) = RB(o);
+    }
+  }
+  const AdjKktSource<true, true> msrc{dv.slot, pv, av, pes, pl, CDG, CSC, dv.delta};
+  const AdjKktSource<false, true> m0{dv.slot, pv, av, pes, pl, CDG, CSC, dv.delta};
+  const AdjKktRhs rhs{dv.n, CStrided{dx, ld, b}, CStrided{rc, ld, b}, CSC};
+  // 3. factor the equilibrated matrix with the forward substitution fused
+  // in, backward sweep (W holds the equilibrated solution S⁻¹ w)
   factor_rows<true>(P, msrc, 0, Strided{lx, ld, b}, Strided{nullptr, 0, 0}, Strided{dinv, ld, b},
                     mywin, rhs, Strided{stash, ld, b});
   const SweepProg SP = sweep_prog(P);
   bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSet{W});
-  // 3. refinement against the exact matrix
+  // 4. refinement against the exact (equilibrated) matrix
   for (int32_t it = 0; it < refine; ++it) {
     for (int32_t i = 0; i < dv.nk; ++i) {
       const int32_t o = P.rperm[i];
@@ -349,7 +383,9 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, lds, lane);
     bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSub{W});
   }
-  // 4. outputs; only real lanes store
+  // the solution w = S W, in place
+  for (int32_t o = 0; o < dv.nk; ++o) W(o) = W(o) * SC(o);
+  // 5. outputs; only real lanes store
   if (b0 >= batch) return;
   const Strided DQ{dq, ld, b}, DL{dl, ld, b}, DU{du, ld, b};
   const int32_t n = dv.n, half = dv.half;
@@ -392,13 +428,17 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
         abx += (av + (int64_t)a2.y * pes)[pl] * W(a2.x);
       }
       const float ax = AX(i);
-      if (t.x >= 0) {  // lower row: G = -A_i, s = l - A x, rhs -dy
+      // a row coupled through c gives b_lambda = ±c/lambda; a decoupled one
+      // (no positive multiplier, or one so small that s/lambda passed -1e30)
+      // gets it from its own equation, G_i b_x + s b_lambda = -dy
+      const bool coupled = DG(i) > kDecoupled;
+      if (t.x >= 0) {  // lower row: G = -A_i, s = l - A x
         const float lam = -nan_min(yv, 0.0f);
-        SOL(base + t.x) = lam > 0.0f ? -c / lam : (-dyv + abx) / (Lo(i) - ax);
+        SOL(base + t.x) = (coupled && lam > 0.0f) ? -c / lam : (-dyv + abx) / (Lo(i) - ax);
       }
-      if (t.y >= 0) {  // upper row: G = A_i, s = A x - u, rhs -dy
+      if (t.y >= 0) {  // upper row: G = A_i, s = A x - u
         const float lam = nan_max(yv, 0.0f);
-        SOL(base + t.y) = lam > 0.0f ? c / lam : (-dyv - abx) / (ax - Up(i));
+        SOL(base + t.y) = (coupled && lam > 0.0f) ? c / lam : (-dyv - abx) / (ax - Up(i));
       }
     }
   }
@@ -628,6 +668,7 @@ int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdl
   WMPC_REQUIRE(n > 0 && m >= 0 && nl >= 0 && nu >= 0 && ne >= 0 && nl <= m && nu <= m && ne <= m,
                "bad dimensions");
   WMPC_REQUIRE(kkt->host.n == nk, "KKT plan dimension must be n + m");
+  WMPC_REQUIRE(nk <= 65535, "the KKT form packs row and column in 16 bits: n + m must be below 65536");
   WMPC_REQUIRE(kkt->host.nnz_a == nz, "nnz_kkt differs from the KKT plan");
   WMPC_REQUIRE(ds->nnz_p >= 0 && ds->nnz_a >= 0 && nz >= 0, "bad counts");
   WMPC_REQUIRE(ds->delta > 0.0, "delta must be positive");
@@ -650,7 +691,6 @@ int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdl
     set[s] = 1;
     slot[4 * s] = kKktP;
     slot[4 * s + 1] = k;
-    slot[4 * s + 2] = 0;
     slot[4 * s + 3] = float_bits(ds->p_diag_sigma[k]);
   }
   for (int32_t k = 0; k < ds->nnz_a; ++k) {
@@ -659,18 +699,17 @@ int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdl
     set[s] = 1;
     slot[4 * s] = kKktA;
     slot[4 * s + 1] = k;
-    slot[4 * s + 2] = 0;
     slot[4 * s + 3] = 0;
   }
   for (int32_t k = 0; k < nz; ++k) {
     const int32_t r = ds->kkt_rows[k], c = ds->kkt_cols[k];
     WMPC_REQUIRE(r >= 0 && c >= 0 && r <= c && c < nk, "KKT entry outside the upper triangle");
+    slot[4 * k + 2] = r | (c << 16);  // row and column of the entry, for the equilibration
     if (set[k]) continue;
     WMPC_REQUIRE(r == c && r >= n, "a KKT slot outside P and A must be a constraint diagonal");
     set[k] = 1;
     slot[4 * k] = kKktDiag;
     slot[4 * k + 1] = r - n;
-    slot[4 * k + 2] = 0;
     slot[4 * k + 3] = 0;
   }
   // rows of the full symmetric KKT (upper entries, then mirrored ones)
@@ -781,7 +820,7 @@ int32_t wmpc_adjoint_is_kkt_form(const wmpc_adjoint_plan* plan) {
 int64_t wmpc_adjoint_work_floats(const wmpc_adjoint_plan* plan, int64_t ld) {
   if (!plan || ld < 0) return -1;
   if (plan->kkt_form)
-    return ((int64_t)plan->adj->host.nnz_l + 4 * (int64_t)plan->k.nk + 3 * (int64_t)plan->k.m) * ld;
+    return ((int64_t)plan->adj->host.nnz_l + 5 * (int64_t)plan->k.nk + 3 * (int64_t)plan->k.m) * ld;
   return ((int64_t)plan->adj->host.nnz_l + 4 * (int64_t)plan->d.dim + 2 * (int64_t)plan->d.ni) * ld;
 }
 
@@ -819,13 +858,14 @@ int wmpc_osqp_adjoint_f32(const wmpc_adjoint_plan* plan, const float* p_values,
     float* dg = wv + nk * ld;
     float* rc = dg + (int64_t)kv.m * ld;
     float* axv = rc + (int64_t)kv.m * ld;
+    float* scv = axv + (int64_t)kv.m * ld;
     if (lds > 64 * 1024)
       WMPC_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_osqp_adjoint_kkt),
                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_osqp_adjoint_kkt, dim3((unsigned)wave_grid(batch, ipw)), dim3(kWave), lds,
                        (hipStream_t)stream, P, kv, p_values, a_values, pm_ld, l, u, x, y, dx, dy, lx,
-                       dinv, stash, rbuf, wv, dg, rc, axv, sol, dq, dl, du, dp, da, refine, batch, ld,
-                       ipw);
+                       dinv, stash, rbuf, wv, dg, rc, axv, scv, sol, dq, dl, du, dp, da, refine, batch,
+                       ld, ipw);
     return check_launch("osqp adjoint (KKT form)");
   }
   const AdjDev& dv = plan->d;

@@ -70,6 +70,7 @@ struct wmpc_osqp_hetero_plan {
   wmpc::HeteroPattern* dev = nullptr;  // device copy of the patterns' tables
   size_t lds = 0;                      // LDS bytes of the launch (largest pattern)
   std::vector<int32_t> order;          // launch order: decreasing nnz_l
+  std::vector<uint64_t> generation;    // of each pattern's KKT programs when copied
 };
 
 namespace wmpc {
@@ -144,6 +145,7 @@ int wmpc_osqp_hetero_plan_create(const wmpc_osqp_plan* const* plans, int32_t cou
   WMPC_REQUIRE(lds <= kMaxLdsBytes, "a pattern needs more LDS than a workgroup can have");
   auto* hp = new wmpc_osqp_hetero_plan();
   hp->plans.assign(plans, plans + count);
+  for (int32_t k = 0; k < count; ++k) hp->generation.push_back(plans[k]->kkt->generation);
   hp->lds = lds;
   hp->order.resize(count);
   std::iota(hp->order.begin(), hp->order.end(), 0);
@@ -189,6 +191,10 @@ int wmpc_osqp_solve_hetero_f32(const wmpc_osqp_hetero_plan* plan, const wmpc_osq
   WMPC_REQUIRE(plan && io && work, "null argument");
   WMPC_REQUIRE(max_iter >= 0, "max_iter must be nonnegative");
   const int32_t count = (int32_t)plan->plans.size();
+  for (int32_t k = 0; k < count; ++k)
+    WMPC_REQUIRE(plan->plans[k]->kkt->generation == plan->generation[k],
+                 "a pattern's programs were re-derived (LDS budget) after the heterogeneous plan was "
+                 "created: create it again");
   std::vector<int64_t> lds(count);
   for (int32_t k = 0; k < count; ++k) lds[k] = io[k].ld;
   WMPC_REQUIRE(work_len >= wmpc_osqp_hetero_work_floats(plan, lds.data()),

@@ -16,4 +16,8 @@ struct wmpc_qdldl_plan {
   void* block = nullptr;
   int device = 0;
   size_t lds_budget = wmpc::kDefaultLdsBytes;
+  int32_t ipw = 0;  // per-plan instances per wavefront (0: process-wide choice)
+  // incremented whenever the programs are re-derived (and their device tables
+  // replaced): holders of a copy of `prog` compare it to detect a stale copy
+  uint64_t generation = 0;
 };

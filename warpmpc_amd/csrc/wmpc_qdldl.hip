@@ -29,7 +29,7 @@ void clear_error() { g_last_error.clear(); }
 
 static std::atomic<int32_t> g_ipw{0};
 int32_t instances_per_wave(int64_t batch, const QdldlProg& P) {
-  return std::min(instances_per_wave(batch), P.lcols);
+  return std::min(P.ipw ? P.ipw : instances_per_wave(batch), P.lcols);
 }
 
 int32_t instances_per_wave(int64_t batch) {
@@ -200,10 +200,13 @@ static int derive(wmpc_qdldl_plan* plan, bool upload = true, bool explicit_budge
     plan->prog.nbw = plan->host.nbw;
     plan->prog.nnz_l = plan->host.nnz_l;
     plan->prog.lcols = lc;
+    plan->prog.ipw = plan->ipw;
     return WMPC_OK;
   }
   const int rc = upload_program(plan);
   plan->prog.lcols = lc;
+  plan->prog.ipw = plan->ipw;
+  ++plan->generation;
   return rc;
 }
 
@@ -439,5 +442,24 @@ int wmpc_set_instances_per_wave(int32_t ipw) {
 }
 
 int32_t wmpc_instances_per_wave(int64_t batch) { return batch < 0 ? -1 : instances_per_wave(batch); }
+
+int wmpc_qdldl_plan_set_instances_per_wave(wmpc_qdldl_plan* plan, int32_t ipw) {
+  clear_error();
+  WMPC_REQUIRE(plan, "null plan");
+  WMPC_REQUIRE(ipw == 0 || ipw == 8 || ipw == 16 || ipw == 32 || ipw == 64,
+               "instances per wavefront must be 0 (process-wide choice), 8, 16, 32 or 64");
+  plan->ipw = ipw;
+  plan->prog.ipw = ipw;
+  return WMPC_OK;
+}
+
+int32_t wmpc_qdldl_plan_instances_per_wave(const wmpc_qdldl_plan* plan, int64_t batch) {
+  if (!plan || batch < 0) return -1;
+  return instances_per_wave(batch, plan->prog);
+}
+
+int64_t wmpc_qdldl_plan_lds_budget(const wmpc_qdldl_plan* plan) {
+  return plan ? (int64_t)plan->lds_budget : -1;
+}
 
 }  // extern "C"

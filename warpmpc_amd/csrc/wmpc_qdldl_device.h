@@ -56,6 +56,8 @@ struct QdldlProg {
   // whose windows do not fit otherwise; lane l uses column l mod lcols, so a
   // launch serves at most lcols instances per wavefront
   int32_t lcols = kWave;
+  // instances per wavefront chosen for this plan (0: the process-wide choice)
+  int32_t ipw = 0;
 };
 
 // LDS slots of the factorization (fused forward adds one window) and the

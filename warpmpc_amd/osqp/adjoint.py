@@ -155,14 +155,14 @@ def _match_broadcast_gradient(grad, reference):
 
 
 def attach_derivatives(compiled: dict, plan: FixedOSQPPlan, refinement_iters: int, form: str = "reference",
-                       kkt_device_plan=None) -> dict:
+                       kkt_device_plan=None, delta: float | None = None) -> dict:
     """Adds the adjoint callables and the differentiable solves to ``compiled``
     (``form``: the adjoint plan kind of ``DeviceAdjointPlan``)."""
     if refinement_iters < 0:
         raise ValueError("derivative_refinement_iters must be nonnegative")
     th = torch()
     lib = _lib.load()
-    dev = DeviceAdjointPlan(plan, form=form, kkt_device_plan=kkt_device_plan)
+    dev = DeviceAdjointPlan(plan, form=form, kkt_device_plan=kkt_device_plan, delta=delta)
     n, m = plan.n, plan.m
     nnz_p, nnz_a = plan.nnz_p, plan.nnz_a
     dp_ = plan.derivative_plan

@@ -131,7 +131,8 @@ def compile_osqp(
     derivative_refinement_iters: int = 100,
     specialize: bool | str = "auto",
     derivative_mode: str = "reference",
-    derivative_kkt_refinement_iters: int = 3,
+    derivative_kkt_refinement_iters: int = 4,
+    derivative_kkt_delta: float = 3e-6,
 ) -> CompiledOSQP:
     """Compile fixed-pattern batched OSQP on the HIP backend (``solver.py:685-737``).
 
@@ -148,7 +149,10 @@ def compile_osqp(
     that refinement converges to wherever the rows with a positive multiplier
     are feasible -- by one factorization of a symmetric system with the
     forward KKT's pattern and ``derivative_kkt_refinement_iters`` refinement
-    solves (see ``wmpc_adjoint_kkt_plan_create``)."""
+    solves (see ``wmpc_adjoint_kkt_plan_create``).  ``derivative_kkt_delta``
+    regularises the constraint diagonal of the per-instance equilibrated
+    system for the float32 factorization: each refinement step contracts the
+    error by about delta * |K⁻¹| of the equilibrated matrix."""
     dtype = np.dtype(dtype)
     settings = plan.settings
     qd = compile_qdldl_variant(
@@ -303,7 +307,7 @@ def compile_osqp(
 
         if derivative_mode == "kkt":
             compiled = attach_derivatives(compiled, plan, derivative_kkt_refinement_iters, form="kkt",
-                                          kkt_device_plan=qd.extras["device_plan"])
+                                          kkt_device_plan=qd.extras["device_plan"], delta=derivative_kkt_delta)
         else:
             compiled = attach_derivatives(compiled, plan, derivative_refinement_iters)
     return CompiledOSQP(**compiled)

@@ -50,6 +50,22 @@ class DevicePlan:
         info = [ctypes.c_int32() for _ in range(4)]
         _lib.check(lib.wmpc_qdldl_plan_info(handle, *[ctypes.byref(v) for v in info]), "wmpc_qdldl_plan_info")
         self.max_reach = int(info[3].value)
+        self.ipw_setting = 0
+
+    def set_instances_per_wave(self, ipw: int) -> None:
+        """Instances per wavefront of this plan's launches (0: the process-wide choice)."""
+        _lib.check(_lib.load().wmpc_qdldl_plan_set_instances_per_wave(self.handle, int(ipw)),
+                   "wmpc_qdldl_plan_set_instances_per_wave")
+        self.ipw_setting = int(ipw)
+
+    def set_lds_budget(self, nbytes: int) -> None:
+        """LDS bytes per workgroup of this plan's kernels (re-derives its programs)."""
+        _lib.check(_lib.load().wmpc_qdldl_plan_set_lds_budget(self.handle, int(nbytes)),
+                   "wmpc_qdldl_plan_set_lds_budget")
+
+    @property
+    def lds_budget(self) -> int:
+        return int(_lib.load().wmpc_qdldl_plan_lds_budget(self.handle))
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -410,7 +410,9 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
                            derivatives: bool = False, derivative_refinement_iters: int = 100,
                            line_search_settings: FilterLineSearchSettings | None = None,
                            group_repeated_stages: bool = True,
-                           derivative_mode: str = "reference") -> CompiledSparseMPCSQP:
+                           derivative_mode: str = "reference",
+                           derivative_kkt_refinement_iters: int = 4,
+                           derivative_kkt_delta: float = 3e-6) -> CompiledSparseMPCSQP:
     """Compile the SQP step functions on the HIP OSQP (sparse_mpc.py:703-1228).
 
     ``derivative_mode`` selects the QP adjoint (``compile_osqp``): the
@@ -430,7 +432,9 @@ def compile_sparse_mpc_sqp(problem: SparseMPCProblem, plan: SparseMPCPlan | None
                       segment_budget=segment_budget, segment_strategy=segment_strategy,
                       level_scheduled_solve=level_scheduled_solve,
                       level_scheduled_solve_threshold=level_scheduled_solve_threshold, derivatives=derivatives,
-                      derivative_refinement_iters=derivative_refinement_iters, derivative_mode=derivative_mode)
+                      derivative_refinement_iters=derivative_refinement_iters, derivative_mode=derivative_mode,
+                      derivative_kkt_refinement_iters=derivative_kkt_refinement_iters,
+                      derivative_kkt_delta=derivative_kkt_delta)
     th = _torch()
     groups = _groups(problem, plan, group_repeated_stages)
     n, m = plan.n_variables, plan.n_constraints
