@@ -137,6 +137,40 @@ int wmpc_qdldl_factor_solve_f32(const wmpc_qdldl_plan* plan, const float* a_valu
                                 void* stream);
 
 /*
+ * float64 variants (the Warp backend's float64 path, warp_backend.py:1211-1224,
+ * with x64 enabled): same layouts as the float32 entry points, one lane per
+ * instance on the plan's tables directly, the operation order of
+ * core.py:867-1028.  work: n * ld doubles (wmpc_qdldl_f64_work_doubles).
+ * wmpc_qdldl_solve_f64: lx_ld == d_ld, 0 = one factor shared by the batch;
+ * x must not alias rhs.
+ */
+int64_t wmpc_qdldl_f64_work_doubles(const wmpc_qdldl_plan* plan, int64_t ld);
+int wmpc_qdldl_factor_f64(const wmpc_qdldl_plan* plan, const double* a_values, double* lx, double* d,
+                          double* dinv, double* work, int64_t work_len, int64_t batch, int64_t ld,
+                          void* stream);
+int wmpc_qdldl_solve_f64(const wmpc_qdldl_plan* plan, const double* lx, int64_t lx_ld,
+                         const double* dinv, int64_t d_ld, const double* rhs, double* x,
+                         double* work, int64_t work_len, int64_t batch, int64_t ld, void* stream);
+
+/*
+ * Level-scheduled solve (CompiledQDLDL.solve with level_scheduled_solve=True:
+ * warp_backend.py:281-454, 970-1076; levels core.py:488-510; schedule
+ * warp_backend.py:1110-1126).  lanes_per_instance (1..64, a power of two)
+ * lanes share one instance: a level with more than `threshold` columns is one
+ * parallel step over those lanes, a run of levels of at most `threshold`
+ * columns one serial step of one lane; the instance's vector lives in LDS
+ * (n * 64 / lanes_per_instance floats per workgroup).  Operation order as the
+ * serial column sweeps (core.py:935-1028).  Arguments as wmpc_qdldl_solve_f32,
+ * without work.  wmpc_qdldl_plan_levels: number of levels of the plan (-1 on
+ * a host-only plan).
+ */
+int32_t wmpc_qdldl_plan_levels(const wmpc_qdldl_plan* plan);
+int wmpc_qdldl_solve_levels_f32(const wmpc_qdldl_plan* plan, int32_t threshold,
+                                int32_t lanes_per_instance, const float* lx, int64_t lx_ld,
+                                const float* dinv, int64_t d_ld, const float* rhs, float* x,
+                                int64_t batch, int64_t ld, void* stream);
+
+/*
  * Fixed-pattern OSQP (warpmpc/jax_osqp/solver.py:685-982).  The descriptor
  * carries the FixedOSQPPlan arrays (types.py:86-132) needed on the device.
  */

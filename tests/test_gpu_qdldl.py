@@ -91,14 +91,15 @@ def test_minimal_example_on_gpu(gpu):
     m = sp.csc_matrix([[4.0, 1.0, 0.0, 0.0], [1.0, 3.0, 0.5, 0.0], [0.0, 0.5, 2.0, 0.25],
                        [0.0, 0.0, 0.25, 1.0]])
     plan = build_qdldl_plan(m)
-    # the flags of examples/jax_qdldl_minimal.py:34-42: accepted, validated, without effect
+    # the flags of examples/jax_qdldl_minimal.py:34-42: segments accepted, validated, without
+    # effect; the level-scheduled solve runs
     qcore._warned_options.clear()
     with pytest.warns(UserWarning, match="no effect"):
         comp = compile_qdldl_variant(plan, np.float32, backend="hip", transpose_work=True, segmented=True,
                                      segment_budget=4, level_scheduled_solve=True,
                                      level_scheduled_solve_threshold=2)
-    assert comp.variant == "hip+transpose"
-    assert comp.extras["options_without_effect"] == {"segmented": True, "level_scheduled_solve": True}
+    assert comp.variant == "hip+transpose+levelsolve"
+    assert comp.extras["options_without_effect"] == {"segmented": True}
     values = np.broadcast_to(plan.a_data.astype(np.float32), (2, plan.nnz_a)).T.copy()
     rhs = np.array([[1.0, 0.0, 0.0, 0.0], [0.0, 1.0, 2.0, 3.0]], dtype=np.float32).T.copy()
     x, _, _ = comp.factor_and_solve(gpu.as_tensor(values).cuda(), gpu.as_tensor(rhs).cuda())

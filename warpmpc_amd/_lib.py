@@ -208,6 +208,12 @@ SIGNATURES: dict[str, tuple] = {
     "wmpc_qdldl_plan_set_instances_per_wave": (ctypes.c_int, [_vp, _i32]),
     "wmpc_qdldl_plan_instances_per_wave": (ctypes.c_int32, [_vp, _i64]),
     "wmpc_qdldl_plan_lds_budget": (ctypes.c_int64, [_vp]),
+    "wmpc_qdldl_f64_work_doubles": (ctypes.c_int64, [_vp, _i64]),
+    "wmpc_qdldl_factor_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "wmpc_qdldl_solve_f64": (ctypes.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _i64, _vp]),
+    "wmpc_qdldl_plan_levels": (ctypes.c_int32, [_vp]),
+    "wmpc_qdldl_solve_levels_f32": (ctypes.c_int, [_vp, _i32, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
+                                                   _vp]),
     "wmpc_adjoint_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointDesc), _vp, ctypes.POINTER(_vp)]),
     "wmpc_adjoint_plan_destroy": (ctypes.c_int, [_vp]),
     "wmpc_adjoint_kkt_plan_create": (ctypes.c_int, [ctypes.POINTER(AdjointKktDesc), _vp, ctypes.POINTER(_vp)]),

@@ -7,6 +7,8 @@
 #include "wmpc_program.h"
 #include "wmpc_qdldl_device.h"
 
+struct wmpc_qdldl_raw;  // tables of the table-direct kernels (wmpc_qdldl_raw.hip)
+
 struct wmpc_qdldl_plan {
   wmpc::QdldlProg prog;    // device programs
   wmpc::HostProgram host;  // host copy of the same programs
@@ -20,4 +22,10 @@ struct wmpc_qdldl_plan {
   // incremented whenever the programs are re-derived (and their device tables
   // replaced): holders of a copy of `prog` compare it to detect a stale copy
   uint64_t generation = 0;
+  wmpc_qdldl_raw* raw = nullptr;  // float64 and level-scheduled kernels' tables
 };
+
+namespace wmpc {
+int raw_upload(const wmpc_qdldl_pattern& pt, wmpc_qdldl_raw** out);
+void raw_destroy(wmpc_qdldl_raw* raw);
+}  // namespace wmpc

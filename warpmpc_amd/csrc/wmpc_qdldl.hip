@@ -281,8 +281,10 @@ static int plan_create(const wmpc_qdldl_pattern* pattern, wmpc_qdldl_plan** out,
     rc = WMPC_ERR_HIP;
   }
   if (rc == WMPC_OK) rc = derive(plan, upload);
+  if (rc == WMPC_OK && upload) rc = raw_upload(plan->pattern, &plan->raw);
   if (rc != WMPC_OK) {
     if (plan->block) (void)hipFree(plan->block);
+    raw_destroy(plan->raw);
     delete plan;
     return rc;
   }
@@ -302,6 +304,7 @@ int wmpc_qdldl_plan_destroy(wmpc_qdldl_plan* plan) {
   clear_error();
   if (!plan) return WMPC_OK;
   if (plan->block) WMPC_HIP_CHECK(hipFree(plan->block));
+  raw_destroy(plan->raw);
   delete plan;
   return WMPC_OK;
 }

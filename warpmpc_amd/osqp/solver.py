@@ -154,6 +154,9 @@ def compile_osqp(
     system for the float32 factorization: each refinement step contracts the
     error by about delta * |K⁻¹| of the equilibrated matrix."""
     dtype = np.dtype(dtype)
+    if dtype != np.dtype(np.float32):
+        raise ValueError("the fused HIP OSQP kernels compute in float32 (float64 is available for the QDLDL "
+                         "factor / solve through compile_qdldl_variant)")
     settings = plan.settings
     qd = compile_qdldl_variant(
         plan.qdldl_plan,
