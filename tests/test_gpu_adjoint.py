@@ -3,7 +3,7 @@
 * Kernel parity: the HIP adjoint and the FP64 adjoint oracle (restatement of
   warpmpc/jax_osqp/solver.py:1047-1192, pinned on the CPU by finite
   differences) are fed the same (x, y, dx, dy); error per instance and output
-  max |v - v64| / max(|v64|_inf, 1), tolerance 1e-4.
+  norm-wise relative (parity_measure.adjoint_errors), tolerance 1e-4.
 * The cart-pole MPC of the differentiable-QP workload (N=30, |u| <= 10,
   Q = diag(exp theta_q), R = exp theta_r, batch 16384), x and y from the HIP
   forward solve, the oracles on sampled instances of the full batch.  With 25
@@ -29,6 +29,7 @@ import pytest
 import scipy.sparse as sp
 
 from oracle import oracle
+from parity_measure import adjoint_errors
 from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
 from warpmpc_amd.problems import OSQP_INFTY, cartpole_dynamics, mpc_qp
 
@@ -39,12 +40,8 @@ INF = 1e30
 HORIZON = 30
 
 
-def _err(v, ref):
-    v = np.asarray(v, dtype=np.float64).reshape(ref.shape[0], -1)
-    ref = np.asarray(ref, dtype=np.float64).reshape(ref.shape[0], -1)
-    if ref.shape[1] == 0:
-        return 0.0
-    return float(np.max(np.max(np.abs(v - ref), axis=1) / np.maximum(np.max(np.abs(ref), axis=1), 1.0)))
+def _errs(got, ref, names=("dq", "dl", "du", "dp", "da")) -> dict:
+    return adjoint_errors(got, ref, names)
 
 
 P3 = np.array([[4.0, 1.0, 0.0], [1.0, 2.0, 0.5], [0.0, 0.5, 3.0]])
@@ -89,9 +86,8 @@ def test_kernel_matches_oracle_on_pinned_cases(gpu, name):
     solver = compile_osqp(plan, np.float32, derivatives=True)
     dq, dl, du, dpm, dam, sol = _fused(solver, pv, av, l, u, xs, ys, dx, dy, want_sol=True)
     ref = _oracle_adjoint(plan, pv, av, l, u, xs, ys, dx, dy)
-    for name_, got, r in zip(("sol", "dq", "dl", "du", "dp", "da"), (sol, dq, dl, du, dpm, dam),
-                             (ref[0], *ref[3:])):
-        assert _err(got, r) < TOL, (name_, _err(got, r))
+    errs = _errs((sol, dq, dl, du, dpm, dam), (ref[0], *ref[3:]), ("sol", "dq", "dl", "du", "dp", "da"))
+    assert max(errs.values()) < TOL, errs
 
 
 def _cartpole_batch(batch, seed, n_h=HORIZON, u_bound=10.0):
@@ -150,9 +146,10 @@ def _cartpole_adjoint_vs_oracle(gpu, plan, solver, b, seed, refine, sample=192):
     adj = oracle.OracleAdjoint(plan, refine=refine)
     r64 = adj.compute(*args)[3:]
     r32 = adj.compute(*args, dtype=np.float32)[3:]
+    e_hip, e_32 = _errs([g[idx] for g in got], r64), _errs(r32, r64)
     errs = {}
-    for name_, g, a64, a32 in zip(("dq", "dl", "du", "dp", "da"), got, r64, r32):
-        errs[name_] = (_err(g[idx], a64), _err(a32, a64))
+    for name_ in e_hip:
+        errs[name_] = (e_hip[name_], e_32[name_])
         print(f"  {name_}: HIP vs FP64 oracle {errs[name_][0]:.2e}   FP32 oracle vs FP64 oracle "
               f"{errs[name_][1]:.2e}", flush=True)
     return errs

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
+from parity_measure import osqp_errors
 from warpmpc_amd import codegen
 from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
 from warpmpc_amd.problems import batched_mpc_data, mass_chain_mpc
@@ -43,25 +44,16 @@ def pair(gpu):
     return prob, plan, gen, ref
 
 
-def _rel(a, b):
-    a = np.atleast_2d(np.asarray(a, dtype=np.float64))
-    b = np.atleast_2d(np.asarray(b, dtype=np.float64))
-    return float(np.max(np.max(np.abs(a - b), axis=-1) / np.maximum(np.max(np.abs(b), axis=-1), 1.0)))
-
-
-def _compare(out_g, out_r, ref64=None, tol_vec=5e-5, tol_scalar=1e-4, tol64=1e-4):
-    names = ("x", "z", "y", "prim_res", "dual_res", "obj_val")
-    for name, g, r in zip(names, out_g, out_r):
-        g, r = g.cpu().numpy(), r.cpu().numpy()
-        if g.ndim == 1:
-            e, tol = float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1.0))), tol_scalar
-        else:
-            e, tol = _rel(g, r), tol_vec
+def _compare(out_g, out_r, q, ref64=None, tol_vec=5e-5, tol_scalar=1e-4, tol64=1e-4):
+    """Generated against generic kernel, and against the FP64 oracle, by the
+    parity tests' measure (norm-wise relative per instance)."""
+    errs = osqp_errors([o.cpu().numpy() for o in out_g], [o.cpu().numpy() for o in out_r], q)
+    for name, e in errs.items():
+        tol = tol_vec if name in ("x", "z", "y") else tol_scalar
         assert e < tol, f"{name}: generated vs generic {e}"
     if ref64 is not None:
-        for name, g, r in zip(names, out_g, ref64):
-            g = g.cpu().numpy()
-            e = float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1.0))) if g.ndim == 1 else _rel(g, r)
+        errs = osqp_errors([o.cpu().numpy() for o in out_g], ref64, q)
+        for name, e in errs.items():
             assert e < tol64, f"{name}: generated vs FP64 {e}"
 
 
@@ -78,7 +70,7 @@ def test_cold_start_matches_generic_and_oracle(pair, batch):
     out_g = gen.solve(pv, av, q, l, u)
     out_r = ref.solve(pv, av, q, l, u)
     ref64 = oracle.OracleOSQP(plan).solve(pv, av, q, l, u)
-    _compare(out_g, out_r, ref64)
+    _compare(out_g, out_r, q, ref64)
 
 
 def test_warm_start_matches_generic_and_oracle(pair):
@@ -91,7 +83,7 @@ def test_warm_start_matches_generic_and_oracle(pair):
     out_g = gen.solve_warm_start(pv, av, q, l, u, *ws)
     out_r = ref.solve_warm_start(pv, av, q, l, u, *ws)
     ref64 = oracle.OracleOSQP(plan).solve(pv, av, q, l, u, x0=ws[0], z0=ws[1], y0=ws[2])
-    _compare(out_g, out_r, ref64)
+    _compare(out_g, out_r, q, ref64)
 
 
 @pytest.mark.parametrize("max_iter", [0, 1])
@@ -103,7 +95,7 @@ def test_few_iterations(gpu, max_iter):
     ref = compile_osqp(plan, np.float32, specialize=False)
     pv, av, q, l, u = batched_mpc_data(prob, 130, seed=3)
     print(f"generated kernel, max_iter {max_iter}", flush=True)
-    _compare(gen.solve(pv, av, q, l, u), ref.solve(pv, av, q, l, u),
+    _compare(gen.solve(pv, av, q, l, u), ref.solve(pv, av, q, l, u), q,
              oracle.OracleOSQP(plan, max_iter=max_iter).solve(pv, av, q, l, u))
 
 
@@ -115,7 +107,7 @@ def test_passed_shared_factor(pair):
     lx, dinv = gen.factor(pv, av)
     out_g = gen.solve_with_factor(lx, dinv, pv, av, q, l, u)
     out_r = ref.solve_with_factor(lx, dinv, pv, av, q, l, u)
-    _compare(out_g, out_r, oracle.OracleOSQP(plan).solve(pv, av, q, l, u))
+    _compare(out_g, out_r, q, oracle.OracleOSQP(plan).solve(pv, av, q, l, u))
 
 
 def test_foreign_code_object_is_refused(pair):

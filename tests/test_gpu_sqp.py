@@ -18,16 +18,12 @@ import numpy as np
 import pytest
 
 from oracle import sqp_oracle
+from parity_measure import rel_err
 from warpmpc_amd.osqp import OSQPSettings
 from warpmpc_amd.sqp import build_sparse_mpc_plan, compile_sparse_mpc_sqp
 from warpmpc_amd.sqp.problems import quadrotor_initial_guess_and_params, quadrotor_problem, tiny_problem
 
 pytestmark = pytest.mark.gpu
-
-
-def _rel(v, ref):
-    v, ref = np.asarray(v, dtype=np.float64), np.asarray(ref, dtype=np.float64)
-    return float(np.max(np.max(np.abs(v - ref), axis=1) / np.maximum(np.max(np.abs(ref), axis=1), 1.0)))
 
 
 def test_tiny_fixed_step_shapes_and_values(gpu):
@@ -43,8 +39,8 @@ def test_tiny_fixed_step_shapes_and_values(gpu):
     assert tuple(nxt.x.shape) == z.shape and tuple(nxt.z.shape) == (2, plan.n_constraints)
     assert bool(gpu.isfinite(result.z_next).all())
     ref = sqp_oracle.OracleSQP(prob, plan).step(z, params, beta=0.25)
-    assert _rel(result.solve.direction.cpu().numpy(), ref["direction"]) < 1e-4
-    assert _rel(result.z_next.cpu().numpy(), ref["z_next"]) < 1e-4
+    assert rel_err(result.solve.direction.cpu().numpy(), ref["direction"]) < 1e-4
+    assert rel_err(result.z_next.cpu().numpy(), ref["z_next"]) < 1e-4
 
 
 def test_tiny_step_derivatives_match_finite_differences(gpu):
@@ -90,13 +86,13 @@ def test_quadrotor_line_search_step_matches_oracle(gpu, quad):
     z, params = quadrotor_initial_guess_and_params(b, seed=4)
     res, _ = sqp.step(z, params, state=sqp.init_state(b))
     ref = sqp_oracle.OracleSQP(prob, plan).step(z, params)
-    e = _rel(res.solve.direction.cpu().numpy(), ref["direction"])
+    e = rel_err(res.solve.direction.cpu().numpy(), ref["direction"])
     print(f"quadrotor step: direction rel err {e:.2e}, steps {res.line_search.step_length.cpu().numpy()} vs "
           f"{ref['step_length']}", flush=True)
     assert e < 2e-3
     np.testing.assert_array_equal(res.line_search.step_length.cpu().numpy(), ref["step_length"])
     np.testing.assert_array_equal(res.line_search.reason.cpu().numpy(), ref["reason"])
-    assert _rel(res.z_next.cpu().numpy(), ref["z_next"]) < 2e-3
+    assert rel_err(res.z_next.cpu().numpy(), ref["z_next"]) < 2e-3
 
 
 def test_quadrotor_ten_iterations(gpu, quad):

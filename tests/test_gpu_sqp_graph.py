@@ -16,16 +16,12 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from parity_measure import rel_err
 from warpmpc_amd.osqp import OSQPSettings
 from warpmpc_amd.sqp import build_sparse_mpc_plan, compile_sparse_mpc_sqp
 from warpmpc_amd.sqp.problems import quadrotor_initial_guess_and_params, quadrotor_problem
 
 pytestmark = pytest.mark.gpu
-
-
-def _rel(v, ref):
-    v, ref = np.asarray(v, dtype=np.float64), np.asarray(ref, dtype=np.float64)
-    return float(np.max(np.max(np.abs(v - ref), axis=1) / np.maximum(np.max(np.abs(ref), axis=1), 1.0)))
 
 
 @pytest.fixture(scope="module")
@@ -68,7 +64,7 @@ def test_replays_equal_eager_iterations(gpu, quad_sqp, batch):
             (scratch @ scratch).sum().item()
         runs.append(got)
         for it, ((gz, gs, gr, gy), (ez, es, er, ey)) in enumerate(zip(got, eager)):
-            ez_err, ey_err = _rel(gz, ez), _rel(gy, ey)
+            ez_err, ey_err = rel_err(gz, ez), rel_err(gy, ey)
             print(f"  run {run} iteration {it}: z vs eager {ez_err:.2e}, warm-start y vs eager {ey_err:.2e}",
                   flush=True)
             assert np.isfinite(gz).all()
@@ -76,7 +72,7 @@ def test_replays_equal_eager_iterations(gpu, quad_sqp, batch):
             np.testing.assert_array_equal(gs, es)
             np.testing.assert_array_equal(gr, er)
     for (a, *_), (b, *_) in zip(*runs):
-        assert _rel(a, b) < 1e-6
+        assert rel_err(a, b) < 1e-6
 
 
 def test_load_validates_shapes(gpu, quad_sqp):
