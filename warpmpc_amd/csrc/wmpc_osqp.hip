@@ -46,9 +46,10 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
                                                        float* __restrict__ dinv, int64_t batch,
                                                        int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
+  // lanes past the batch factor the last instance again (identical values to
+  // the same addresses): the table streams need the whole wavefront
   int64_t b0;
   const int64_t b = lane_instance(ipw, batch, &b0);
-  if (b0 >= batch) return;
   const int64_t pb = src.pm_ld ? b : 0;
   factor_rows<false>(P, src, pb, Strided{lx, ld, b}, Strided{nullptr, 0, 0},
                      Strided{dinv, ld, b}, lds + (threadIdx.x & (P.lcols - 1)), NoRhs{},

@@ -49,9 +49,10 @@ __global__ __launch_bounds__(kWave) void k_qdldl_factor(QdldlProg P, DirectSourc
                                                         float* __restrict__ dinv, int64_t batch,
                                                         int64_t ld, int32_t ipw) {
   extern __shared__ float lds[];
+  // lanes past the batch factor the last instance again (identical values to
+  // the same addresses): the table streams need the whole wavefront
   int64_t b0;
   const int64_t b = lane_instance(ipw, batch, &b0);
-  if (b0 >= batch) return;
   factor_rows<false>(P, src, b, Strided{lx, ld, b}, Strided{d, ld, b}, Strided{dinv, ld, b},
                      lds + (threadIdx.x & (P.lcols - 1)), NoRhs{}, Strided{nullptr, 0, 0});
 }
@@ -139,6 +140,9 @@ static int upload_program(wmpc_qdldl_plan* plan) {
   P.ncol = h.ncol;
   P.nact = h.nact;
   P.nbw = h.nbw;
+  P.ninit = (int32_t)(h.init.size() / 2);
+  P.nproc = (int32_t)(h.proc.size() / 4);
+  P.nupd = (int32_t)(h.upd.size() / 2);
   size_t i = 0;
   P.row = reinterpret_cast<const int4*>(base + offs[i++]);
   P.init = reinterpret_cast<const int2*>(base + offs[i++]);

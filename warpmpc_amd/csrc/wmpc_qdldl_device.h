@@ -3,7 +3,9 @@
 // One lane owns one instance; one wavefront (64 lanes) is one workgroup.  Every
 // per-instance array in global memory is batch-major ([entry][ld]), so a
 // wavefront touching entry i of 64 consecutive instances issues one contiguous
-// 256-byte access.  Program tables are wave-uniform (scalar loads).  Working
+// 256-byte access.  Program tables are wave-uniform: the factorization reads
+// them as streams of 64-entry chunks (one coalesced load a chunk ahead, an
+// entry per v_readlane), so it needs whole wavefronts.  Working
 // values live in LDS slots laid out [slot][64]: lane t of slot s is
 // lds[s * 64 + t], so one slot access of a wavefront hits 64 distinct banks.
 //
@@ -58,6 +60,46 @@ struct QdldlProg {
   int32_t lcols = kWave;
   // instances per wavefront chosen for this plan (0: the process-wide choice)
   int32_t ipw = 0;
+  // entries of the init / proc / upd tables
+  int32_t ninit = 0, nproc = 0, nupd = 0;
+};
+
+__device__ __forceinline__ int2 read_lane(int2 v, int32_t j) {
+  return make_int2(__builtin_amdgcn_readlane(v.x, j), __builtin_amdgcn_readlane(v.y, j));
+}
+__device__ __forceinline__ int4 read_lane(int4 v, int32_t j) {
+  return make_int4(__builtin_amdgcn_readlane(v.x, j), __builtin_amdgcn_readlane(v.y, j),
+                   __builtin_amdgcn_readlane(v.z, j), __builtin_amdgcn_readlane(v.w, j));
+}
+
+// Sequential reader of a wave-uniform table of `count` entries: lane l holds
+// entry base + l of the current 64-entry chunk and base + 64 + l of the next,
+// so one coalesced load, issued a chunk ahead, serves 64 reads, and an entry
+// reaches scalar registers by v_readlane instead of a load per read.  Reads
+// must come in nondecreasing order, and every lane of the wavefront must be
+// active (v_readlane reads lanes regardless of the execution mask).
+template <class T>
+struct TableStream {
+  const T* t;
+  int32_t count, base;
+  T cur, nxt;
+  __device__ __forceinline__ T fetch(int32_t e) const {
+    return e < count ? t[e] : T{};
+  }
+  __device__ __forceinline__ TableStream(const T* table, int32_t n, int32_t first)
+      : t(table), count(n), base(first) {
+    const int32_t l = (int32_t)(threadIdx.x & (kWave - 1));
+    cur = fetch(first + l);
+    nxt = fetch(first + kWave + l);
+  }
+  __device__ __forceinline__ T operator[](int32_t i) {
+    while (i - base >= kWave) {
+      base += kWave;
+      cur = nxt;
+      nxt = fetch(base + kWave + (int32_t)(threadIdx.x & (kWave - 1)));
+    }
+    return read_lane(cur, i - base);
+  }
 };
 
 // LDS slots of the factorization (fused forward adds one window) and the
@@ -128,7 +170,8 @@ struct NoRhs {
 // lx / d / dinv: outputs (d may have ptr == nullptr).  lds: this lane's slot
 // column base.  FWD additionally runs the forward substitution
 // w[k] = rhs(k) - sum_c L[k,c] w[c] with the freshly computed row of L and
-// stores w into stash(k).
+// stores w into stash(k).  Called by every lane of the wavefront together
+// (the program tables are read through TableStream).
 template <bool FWD, class Src, class Rhs>
 __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, int64_t sb,
                                             const Strided& lx, const Strided& d,
@@ -139,25 +182,40 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
   float* CW = Y + P.ymax * lc;
   float* AL = CW + P.ncol * lc;
   float* FW = AL + P.nact * lc;
+  TableStream<int4> rows(P.row, P.n, 0), procs(P.proc, P.nproc, 0);
+  TableStream<int2> inits(P.init, P.ninit, 0), upds(P.upd, P.nupd, 0);
   int32_t r = 0, j = 0, u = 0;
   for (int32_t k = 0; k < P.n; ++k) {
-    const int4 ri = P.row[k];
+    const int4 ri = rows[k];
     const int32_t r1 = ri.x;
     for (int32_t s = 0; s < r1 - r; ++s) Y[s * lc] = 0.f;
-    for (; j < ri.y; ++j) {
-      const int2 e = P.init[j];
-      Y[e.x * lc] += src(e.y, sb);
-    }
     float dk = src(ri.w, sb);
     float fw = 0.f;
     if constexpr (FWD) fw = rhs(k, P.p[k]);
+    // the column's values are independent loads: issue them in groups of 8
+    // before their (in-order) accumulation into the row window
+    for (; j + 8 <= ri.y; j += 8) {
+      int2 e[8];
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        e[i] = inits[j + i];
+        v[i] = src(e[i].y, sb);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Y[e[i].x * lc] += v[i];
+    }
+    for (; j < ri.y; ++j) {
+      const int2 e = inits[j];
+      Y[e.x * lc] += src(e.y, sb);
+    }
     for (int32_t s = 0; r < r1; ++r, ++s) {
-      const int4 pe = P.proc[r];
+      const int4 pe = procs[r];
       const float val = Y[s * lc];
       // the updates of one reach entry hit distinct later slots: issue their
       // loads in groups so several LDS / global round trips overlap
       for (; u + 4 <= pe.w; u += 4) {
-        const int2 e0 = P.upd[u], e1 = P.upd[u + 1], e2 = P.upd[u + 2], e3 = P.upd[u + 3];
+        const int2 e0 = upds[u], e1 = upds[u + 1], e2 = upds[u + 2], e3 = upds[u + 3];
         const float l0 = e0.y >= 0 ? AL[e0.y * lc] : lx(-(int64_t)e0.y - 1);
         const float l1 = e1.y >= 0 ? AL[e1.y * lc] : lx(-(int64_t)e1.y - 1);
         const float l2 = e2.y >= 0 ? AL[e2.y * lc] : lx(-(int64_t)e2.y - 1);
@@ -170,7 +228,7 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
         Y[e3.x * lc] = y3 - l3 * val;
       }
       for (; u < pe.w; ++u) {
-        const int2 e = P.upd[u];
+        const int2 e = upds[u];
         const float l = e.y >= 0 ? AL[e.y * lc] : lx(-(int64_t)e.y - 1);
         Y[e.x * lc] -= l * val;
       }
