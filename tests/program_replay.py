@@ -218,10 +218,15 @@ def replay_factor_solve(hp: HostPlan, a: np.ndarray, rhs: np.ndarray):
         fw = rhs[p[k]]
         t = 0
         while r < r1:
-            cs, lidx, keep, uend = (int(v) for v in hp.proc[r])
+            packed, lidx, split, uend = (int(v) for v in hp.proc[r])
+            cs, keep = packed & 0xFFFF, (packed >> 16) - 1
+            if not u <= split <= uend:
+                raise AssertionError(f"reach entry {r}: global split {split} outside [{u}, {uend}]")
             val = Y[t]
             while u < uend:
                 ys, ls = (int(v) for v in hp.upd[u])
+                if (ls >= 0) != (u < split):
+                    raise AssertionError(f"update {u}: L source on the wrong side of the split {split}")
                 lv = AL[ls] if ls >= 0 else lx[-ls - 1]
                 if np.isnan(lv):
                     raise AssertionError(f"update {u} reads L slot {-ls - 1} before it is written")

@@ -64,9 +64,12 @@ struct HostProgram {
   // row k: {proc_end, init_end, colslot of k (-1 if column k is never reached), diag value slot}
   std::vector<int32_t> row;   // 4 per row
   std::vector<int32_t> init;  // 2 per entry: {yslot, original value slot}
-  // reach entry r (row k, column c): {colslot(c), lidx of L[k,c], AL slot kept (-1), update end}
+  // reach entry r (row k, column c): {colslot(c) | (AL slot kept + 1) << 16 (0: not kept),
+  // lidx of L[k,c], first update whose L comes from global memory, update end}
   std::vector<int32_t> proc;  // 4 per entry
-  // update u: {target yslot, L source: AL slot >= 0, or -(lidx+1) for global}
+  // update u: {target yslot, L source: AL slot >= 0, or -(lidx+1) for global}; the
+  // updates of one reach entry hit distinct slots, so their order is free: those
+  // reading a live-L slot come first, then the global re-reads
   std::vector<int32_t> upd;   // 2 per entry
   // column c (backward order): {bslot of c (-1 if row c has no entries), lp[c+1]}
   std::vector<int32_t> bcol;  // 2 per column
@@ -238,13 +241,23 @@ inline std::string build_program(const wmpc_qdldl_pattern* pt, int32_t act_cap, 
       if (pt->proc_lidx[r] != pt->lp[c] + off) return "proc_lidx must be lp[c] + offset";
       if (pt->li[pt->lp[c] + off] != k) return "L slot of a reach entry must hold row k";
       for (int32_t qq = 0; qq < off; ++qq) {
-        const int32_t lidx = pt->lp[c] + qq;
-        const int32_t row = pt->li[lidx];
+        const int32_t row = pt->li[pt->lp[c] + qq];
         if (row >= k || stamp[row] != k || slot_of[row] <= r - r0)
           return "update target must be a later reach slot";
-        hp.upd.push_back(slot_of[row]);
-        hp.upd.push_back(act_of[lidx] >= 0 ? act_of[lidx] : -(lidx + 1));
-        if (act_of[lidx] < 0) ++hp.global_rereads;
+      }
+      for (int32_t qq = 0; qq < off; ++qq) {  // live-L operands
+        const int32_t lidx = pt->lp[c] + qq;
+        if (act_of[lidx] < 0) continue;
+        hp.upd.push_back(slot_of[pt->li[lidx]]);
+        hp.upd.push_back(act_of[lidx]);
+      }
+      const int32_t split = (int32_t)(hp.upd.size() / 2);
+      for (int32_t qq = 0; qq < off; ++qq) {  // L entries re-read from global memory
+        const int32_t lidx = pt->lp[c] + qq;
+        if (act_of[lidx] >= 0) continue;
+        hp.upd.push_back(slot_of[pt->li[lidx]]);
+        hp.upd.push_back(-(lidx + 1));
+        ++hp.global_rereads;
       }
       int32_t keep = -1;
       const int32_t lidx = pt->proc_lidx[r];
@@ -253,10 +266,11 @@ inline std::string build_program(const wmpc_qdldl_pattern* pt, int32_t act_cap, 
         keep = actpool.take();
         act_of[lidx] = keep;
       }
+      if (colslot[c] > 0xFFFF || keep >= 0xFFFF) return "window slot beyond the 16-bit record fields";
       int32_t* pe = &hp.proc[4 * (size_t)r];
-      pe[0] = colslot[c];
+      pe[0] = colslot[c] | ((keep + 1) << 16);
       pe[1] = lidx;
-      pe[2] = keep;
+      pe[2] = split;
       pe[3] = (int32_t)(hp.upd.size() / 2);
     }
     act_need = std::max(act_need, act_live);

@@ -39,7 +39,7 @@ struct QdldlProg {
   int32_t ymax = 0, ncol = 0, nact = 0, nbw = 0;
   const int4* row = nullptr;   // [n] {proc_end, init_end, colslot_k, diag_src}
   const int2* init = nullptr;  // {yslot, src}
-  const int4* proc = nullptr;  // {colslot(c), lidx, AL slot kept or -1, upd_end}
+  const int4* proc = nullptr;  // {colslot(c) | (AL slot kept + 1) << 16, lidx, first global update, upd_end}
   const int2* upd = nullptr;   // {yslot, AL slot or -(lidx+1)}
   const int2* bcol = nullptr;  // [n] {bslot_c or -1, lp[c+1]}
   const int32_t* bent = nullptr;  // [nnz_l]
@@ -213,13 +213,27 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
       const int4 pe = procs[r];
       const float val = Y[s * lc];
       // the updates of one reach entry hit distinct later slots: issue their
-      // loads in groups so several LDS / global round trips overlap
+      // loads in groups so several LDS / global round trips overlap; the
+      // live-L operands (LDS) come first, then the L entries re-read from
+      // global memory, so neither loop branches on the operand's source
+      for (; u + 4 <= pe.z; u += 4) {
+        const int2 e0 = upds[u], e1 = upds[u + 1], e2 = upds[u + 2], e3 = upds[u + 3];
+        const float l0 = AL[e0.y * lc], l1 = AL[e1.y * lc], l2 = AL[e2.y * lc], l3 = AL[e3.y * lc];
+        const float y0 = Y[e0.x * lc], y1 = Y[e1.x * lc];
+        const float y2 = Y[e2.x * lc], y3 = Y[e3.x * lc];
+        Y[e0.x * lc] = y0 - l0 * val;
+        Y[e1.x * lc] = y1 - l1 * val;
+        Y[e2.x * lc] = y2 - l2 * val;
+        Y[e3.x * lc] = y3 - l3 * val;
+      }
+      for (; u < pe.z; ++u) {
+        const int2 e = upds[u];
+        Y[e.x * lc] -= AL[e.y * lc] * val;
+      }
       for (; u + 4 <= pe.w; u += 4) {
         const int2 e0 = upds[u], e1 = upds[u + 1], e2 = upds[u + 2], e3 = upds[u + 3];
-        const float l0 = e0.y >= 0 ? AL[e0.y * lc] : lx(-(int64_t)e0.y - 1);
-        const float l1 = e1.y >= 0 ? AL[e1.y * lc] : lx(-(int64_t)e1.y - 1);
-        const float l2 = e2.y >= 0 ? AL[e2.y * lc] : lx(-(int64_t)e2.y - 1);
-        const float l3 = e3.y >= 0 ? AL[e3.y * lc] : lx(-(int64_t)e3.y - 1);
+        const float l0 = lx(-(int64_t)e0.y - 1), l1 = lx(-(int64_t)e1.y - 1);
+        const float l2 = lx(-(int64_t)e2.y - 1), l3 = lx(-(int64_t)e3.y - 1);
         const float y0 = Y[e0.x * lc], y1 = Y[e1.x * lc];
         const float y2 = Y[e2.x * lc], y3 = Y[e3.x * lc];
         Y[e0.x * lc] = y0 - l0 * val;
@@ -229,14 +243,14 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
       }
       for (; u < pe.w; ++u) {
         const int2 e = upds[u];
-        const float l = e.y >= 0 ? AL[e.y * lc] : lx(-(int64_t)e.y - 1);
-        Y[e.x * lc] -= l * val;
+        Y[e.x * lc] -= lx(-(int64_t)e.y - 1) * val;
       }
-      const float lval = val * CW[pe.x * lc];
+      const int32_t cs = pe.x & 0xFFFF, keep = (pe.x >> 16) - 1;
+      const float lval = val * CW[cs * lc];
       lx(pe.y) = lval;
-      if (pe.z >= 0) AL[pe.z * lc] = lval;
+      if (keep >= 0) AL[keep * lc] = lval;
       dk -= val * lval;
-      if constexpr (FWD) fw -= lval * FW[pe.x * lc];
+      if constexpr (FWD) fw -= lval * FW[cs * lc];
     }
     if (d.ptr) d(k) = dk;
     const float di = 1.0f / dk;
