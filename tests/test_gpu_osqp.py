@@ -263,3 +263,28 @@ def test_work_size_is_exactly_the_documented_one(gpu, chain):
     np.testing.assert_array_equal(outs[0].t().cpu().numpy(), ref[0].cpu().numpy())
     assert lib.wmpc_osqp_solve_f32(*args(need - 1)) != 0
     assert "wmpc_osqp_work_floats" in lib.wmpc_last_error().decode()
+
+
+@pytest.mark.parametrize("specialize", [False, "auto"])
+def test_nan_input_propagates_per_instance(gpu, specialize):
+    """A NaN in one instance's q reaches that instance's x, z, y, residuals and
+    objective (the box projection and the residual maxima propagate NaN like
+    jnp.minimum / jnp.maximum / jnp.max, solver.py:818, 866-870); the other
+    instances match the oracle.  Generic record-stream kernel and, where
+    built, the generated kernel of the pattern."""
+    prob = mass_chain_mpc(horizon=10)
+    s = OSQPSettings(rho=0.1, sigma=1e-6, alpha=1.6, max_iter=25, rho_is_vec=True)
+    plan = build_osqp_plan(prob.p_matrix, prob.a_matrix, prob.l, prob.u, s)
+    solver = compile_osqp(plan, np.float32, specialize=specialize)
+    b = 70
+    pv, av, q, l, u = batched_mpc_data(prob, b, seed=12)
+    q = q.copy()
+    q[3, 5] = np.nan
+    out = [o.cpu().numpy() for o in solver.solve(pv, av, q, l, u)]
+    for o in out:
+        assert np.isnan(o[3]).all(), o[3]
+    keep = np.arange(b) != 3
+    ref = oracle.OracleOSQP(plan).solve(pv, av, q, l, u)
+    assert all(np.isnan(r[3]).all() for r in ref)
+    errs = osqp_errors([o[keep] for o in out], [r[keep] for r in ref], q[keep])
+    assert max(errs.values()) < TOL, errs
