@@ -95,15 +95,16 @@ struct StashOut {
 // One pass over the iterate (solver.py:806-820): unless FIRST, the relaxation,
 // projection and dual update with the KKT solution S; unless LAST, the next
 // right-hand side R (solver.py:808-810) at the entry's first-touch position.
+// The entry tables are read as streams (TableStream): every lane takes part.
 template <bool FIRST, bool LAST>
 __device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, const Strided& R,
                                           const Strided& X, const Strided& Z, const Strided& Y,
                                           const CStrided& Q, const CStrided& Lo,
                                           const CStrided& Up) {
   const float alpha = dv.alpha, oma = 1.0f - dv.alpha, sigma = dv.sigma;
-#pragma unroll 4
+  TableStream<int4> vt(dv.vtab, dv.nvar, 0);
   for (int32_t v = 0; v < dv.nvar; ++v) {
-    const int4 t = dv.vtab[v];
+    const int4 t = vt[v];
     const float xo = X(t.y);
     float xn = xo;
     if constexpr (!FIRST) {
@@ -112,14 +113,14 @@ __device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, 
     }
     if constexpr (!LAST) R(t.z) = sigma * xn - __int_as_float(t.w) * Q(t.y);
   }
-#pragma unroll 4
+  TableStream<int4> ct(dv.ctab, 2 * dv.ncon, 0);
   for (int32_t k = 0; k < dv.ncon; ++k) {
-    const int4 t = dv.ctab[2 * k];
+    const int4 t = ct[2 * k];
     const float rinv = __int_as_float(t.w);
     const float zo = Z(t.y), yo = Y(t.y);
     float zn = zo, yn = yo;
     if constexpr (!FIRST) {
-      const int4 t2 = dv.ctab[2 * k + 1];
+      const int4 t2 = ct[2 * k + 1];
       const float rho = __int_as_float(t2.x), e = __int_as_float(t2.y);
       const float rz = zo - rinv * yo;
       const float zt = rz + rinv * S(t.x);
