@@ -67,6 +67,13 @@ struct AdjKktDev {
   const int4* con = nullptr;  // [m] {lower position k or -1, upper position k or -1, eq position or -1, 0}
   const int2* pent = nullptr;  // [nnz_p] {row, col}
   const int2* aent = nullptr;  // [nnz_a] {row, col}
+  // the rows of the full symmetric KKT again, in the sweeps' first-touch order
+  // with each entry's value source inlined, so the residual loops read two
+  // sequential streams: rows {original row, end of its entries}, entries
+  // {col, kind, index, perturbed offset bits}
+  int32_t nmv = 0;
+  const int2* mrow = nullptr;  // [nk]
+  const int4* ment = nullptr;  // [nmv]
 };
 }  // namespace wmpc
 
@@ -273,12 +280,22 @@ struct AdjKktSource {
     const int4 t = slot[s];
     float f = 1.0f;
     if constexpr (SCALED) f = sc(t.z & 0xffff) * sc((uint32_t)t.z >> 16);
-    if (t.x == kKktP) {
-      const float v = (p + (int64_t)t.y * pes)[pl];
-      return (PERT ? v + __int_as_float(t.w) : v) * f;
+    return from(t.x, t.y, t.w, f);
+  }
+  // Entry {col, kind, index, perturbed offset bits} of row o (the scale
+  // product is commutative, so sc(o) sc(col) equals the upper entry's).
+  __device__ __forceinline__ float entry(const int4 e, int32_t o) const {
+    float f = 1.0f;
+    if constexpr (SCALED) f = sc(o) * sc(e.x);
+    return from(e.y, e.z, e.w, f);
+  }
+  __device__ __forceinline__ float from(int32_t kind, int32_t idx, int32_t pert, float f) const {
+    if (kind == kKktP) {
+      const float v = (p + (int64_t)idx * pes)[pl];
+      return (PERT ? v + __int_as_float(pert) : v) * f;
     }
-    if (t.x == kKktA) return (a + (int64_t)t.y * pes)[pl] * f;
-    const float d = dg(t.y);
+    if (kind == kKktA) return (a + (int64_t)idx * pes)[pl] * f;
+    const float d = dg(idx);
     if (d <= kDecoupled) return d;
     return PERT ? d * f - delta : d * f;
   }
@@ -347,11 +364,18 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     const AdjKktSource<false, false> raw{dv.slot, pv, av, pes, pl, CDG, CSC, dv.delta};
     for (int32_t o = 0; o < dv.nk; ++o) SC(o) = 1.0f;
     for (int32_t pass = 0; pass < kRuizPasses; ++pass) {
-      for (int32_t o = 0; o < dv.nk; ++o) {
+      // rows in first-touch order (each row's scale depends only on the
+      // previous pass)
+      TableStream<int2> rows(dv.mrow, dv.nk, 0);
+      TableStream<int4> ents(dv.ment, dv.nmv, 0);
+      int32_t e = 0;
+      for (int32_t i = 0; i < dv.nk; ++i) {
+        const int2 r = rows[i];
+        const int32_t o = r.x;
         float mx = 0.0f;
-        for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
-          const int2 t = dv.mv[e];
-          const float v = raw(t.y);
+        for (; e < r.y; ++e) {
+          const int4 t = ents[e];
+          const float v = raw.entry(t, o);
           if (v > kDecoupled) mx = fmaxf(mx, fabsf(v) * SC(t.x));
         }
         mx *= SC(o);
@@ -371,14 +395,17 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
   bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSet{W});
   // 4. refinement against the exact (equilibrated) matrix
   for (int32_t it = 0; it < refine; ++it) {
+    TableStream<int2> rows(dv.mrow, dv.nk, 0);
+    TableStream<int4> ents(dv.ment, dv.nmv, 0);
+    int32_t e = 0;
     for (int32_t i = 0; i < dv.nk; ++i) {
-      const int32_t o = P.rperm[i];
+      const int2 r = rows[i];
       float s = 0.0f;
-      for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
-        const int2 t = dv.mv[e];
-        s += m0(t.y) * W(t.x);
+      for (; e < r.y; ++e) {
+        const int4 t = ents[e];
+        s += m0.entry(t, r.x) * W(t.x);
       }
-      RB(i) = s - rhs.at(o);
+      RB(i) = s - rhs.at(r.x);
     }
     fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, lds, lane);
     bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSub{W});
@@ -772,10 +799,27 @@ int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdl
     aent[2 * k] = ds->a_rows[k];
     aent[2 * k + 1] = ds->a_cols[k];
   }
+  // the same rows in the sweeps' first-touch order, value sources inlined
+  const std::vector<int32_t>& rperm = kkt->host.rperm;
+  WMPC_REQUIRE((int32_t)rperm.size() == nk, "KKT plan has no first-touch order");
+  std::vector<int32_t> mrow(2 * (size_t)nk), ment;
+  ment.reserve(2 * mv.size());
+  for (int32_t i = 0; i < nk; ++i) {
+    const int32_t o = rperm[i];
+    WMPC_REQUIRE(o >= 0 && o < nk, "first-touch order out of range");
+    for (int32_t e = mv_ptr[o]; e < mv_ptr[o + 1]; ++e) {
+      const int32_t k = mv[2 * (size_t)e + 1];
+      ment.insert(ment.end(), {mv[2 * (size_t)e], slot[4 * (size_t)k], slot[4 * (size_t)k + 1],
+                               slot[4 * (size_t)k + 3]});
+    }
+    mrow[2 * (size_t)i] = o;
+    mrow[2 * (size_t)i + 1] = (int32_t)(ment.size() / 4);
+  }
   Blob blob;
   const size_t t_slot = blob.add(slot), t_mvp = blob.add(mv_ptr), t_mv = blob.add(mv),
                t_arp = blob.add(arow_ptr), t_ar = blob.add(arow), t_con = blob.add(con),
-               t_pe = blob.add(pent), t_ae = blob.add(aent);
+               t_pe = blob.add(pent), t_ae = blob.add(aent), t_mr = blob.add(mrow),
+               t_me = blob.add(ment);
   auto* plan = new wmpc_adjoint_plan();
   if (int rc = blob.upload(&plan->block, "adjoint KKT plan")) {
     delete plan;
@@ -800,6 +844,9 @@ int wmpc_adjoint_kkt_plan_create(const wmpc_adjoint_kkt_desc* ds, const wmpc_qdl
   d.con = blob.at<int4>(t_con);
   d.pent = blob.at<int2>(t_pe);
   d.aent = blob.at<int2>(t_ae);
+  d.nmv = (int32_t)(ment.size() / 4);
+  d.mrow = blob.at<int2>(t_mr);
+  d.ment = blob.at<int4>(t_me);
   // the M0 width the reference's adjoint_derivative_compute returns
   plan->d.n = n;
   plan->d.m = m;
