@@ -49,6 +49,13 @@ struct AdjDev {
   const int4* con = nullptr;       // [m] {lower, upper, equality position in sol or -1, 0}
   const int2* pent = nullptr;      // [nnz_p] {row, col}
   const int2* aent = nullptr;      // [nnz_a] {row, col}
+  // the rows of M0 again, in the sweeps' first-touch order with the value
+  // and right-hand-side sources inlined, so the refinement's residual reads
+  // two sequential streams: rows {original row, end of its entries, rhs kind,
+  // rhs index}, entries {col | kind << 28, i0, i1, f bits}
+  int32_t nmv = 0;
+  const int4* mrow = nullptr;  // [dim]
+  const int4* ment = nullptr;  // [nmv]
 };
 
 }  // namespace wmpc
@@ -99,17 +106,24 @@ struct AdjSource {
   CStrided lam, slk;
   __device__ __forceinline__ float operator()(int32_t s, int64_t = 0) const {
     const int4 t = slot[s];
-    switch (t.x) {
+    return from(t.x, t.y, t.z, t.w);
+  }
+  // Entry {col | kind << 28, i0, i1, f bits} of a row stream.
+  __device__ __forceinline__ float entry(const int4 e) const {
+    return from((int32_t)((uint32_t)e.x >> 28), e.y, e.z, e.w);
+  }
+  __device__ __forceinline__ float from(int32_t kind, int32_t i0, int32_t i1, int32_t fb) const {
+    switch (kind) {
       case kSlotConst:
-        return __int_as_float(PERT ? t.y : t.w);
+        return __int_as_float(PERT ? i0 : fb);
       case kSlotP:
-        return (p + (int64_t)t.y * pes)[pl];
+        return (p + (int64_t)i0 * pes)[pl];
       case kSlotA:
-        return (a + (int64_t)t.y * pes)[pl] * __int_as_float(t.w);
+        return (a + (int64_t)i0 * pes)[pl] * __int_as_float(fb);
       case kSlotGLambda:
-        return lam(t.z) * ((a + (int64_t)t.y * pes)[pl] * __int_as_float(t.w));
+        return lam(i1) * ((a + (int64_t)i0 * pes)[pl] * __int_as_float(fb));
       default:
-        return slk(t.y);
+        return slk(i0);
     }
   }
 };
@@ -120,11 +134,14 @@ struct AdjRhs {
   CStrided dx, dy, y;
   __device__ __forceinline__ float at(int32_t o) const {
     const int2 r = rsrc[o];
-    if (r.x == kRhsDx) return -dx(r.y);
-    if (r.x == kRhsDy) return -dy(r.y);
-    if (r.x == kRhsEq) {
-      const float d = dy(r.y);
-      return -(y(r.y) >= 0.0f ? d : -d);
+    return from(r.x, r.y);
+  }
+  __device__ __forceinline__ float from(int32_t kind, int32_t idx) const {
+    if (kind == kRhsDx) return -dx(idx);
+    if (kind == kRhsDy) return -dy(idx);
+    if (kind == kRhsEq) {
+      const float d = dy(idx);
+      return -(y(idx) >= 0.0f ? d : -d);
     }
     return 0.0f;
   }
@@ -187,14 +204,16 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
   bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSet{SOL});
   for (int32_t it = 0; it < refine; ++it) {
     // residual M0 sol - rhs, staged in the sweep's first-touch order
+    TableStream<int4> rows(dv.mrow, dv.dim, 0), ents(dv.ment, dv.nmv, 0);
+    int32_t e = 0;
     for (int32_t i = 0; i < dv.dim; ++i) {
-      const int32_t o = P.rperm[i];
+      const int4 r = rows[i];
       float s = 0.0f;
-      for (int32_t e = dv.mv_ptr[o]; e < dv.mv_ptr[o + 1]; ++e) {
-        const int2 t = dv.mv[e];
-        s += m0(t.y) * SOL(t.x);
+      for (; e < r.y; ++e) {
+        const int4 t = ents[e];
+        s += m0.entry(t) * SOL(t.x & 0x0FFFFFFF);
       }
-      RB(i) = s - rhs.at(o);
+      RB(i) = s - rhs.from(r.z, r.w);
     }
     fwd_cols<false>(SP, lx, dinv, ld, b, rbuf, stash, ld, b, lds, lane);
     bwd_cols<false>(SP, lx, ld, b, stash, ld, b, lds, lane, SolSub{SOL});
@@ -645,9 +664,31 @@ int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* ds, const wmpc_qdldl_plan*
     aent[2 * k + 1] = ds->a_cols[k];
   }
   Blob blob;
+  // the rows of M0 in the sweeps' first-touch order, sources inlined
+  const std::vector<int32_t>& rperm = adj->host.rperm;
+  WMPC_REQUIRE((int32_t)rperm.size() == dim, "adjoint plan has no first-touch order");
+  WMPC_REQUIRE(dim < (1 << 28), "the adjoint row stream packs columns in 28 bits");
+  std::vector<int32_t> mrow(4 * (size_t)dim), ment;
+  ment.reserve(2 * mv.size());
+  for (int32_t i = 0; i < dim; ++i) {
+    const int32_t o = rperm[i];
+    WMPC_REQUIRE(o >= 0 && o < dim, "first-touch order out of range");
+    for (int32_t e = mv_ptr[o]; e < mv_ptr[o + 1]; ++e) {
+      const int32_t k = mv[2 * (size_t)e + 1];
+      const int32_t kind = slot[4 * (size_t)k];
+      WMPC_REQUIRE(kind >= 0 && kind < 8, "slot kind out of range");
+      ment.insert(ment.end(), {mv[2 * (size_t)e] | (kind << 28), slot[4 * (size_t)k + 1],
+                               slot[4 * (size_t)k + 2], slot[4 * (size_t)k + 3]});
+    }
+    mrow[4 * (size_t)i] = o;
+    mrow[4 * (size_t)i + 1] = (int32_t)(ment.size() / 4);
+    mrow[4 * (size_t)i + 2] = rsrc[2 * (size_t)o];
+    mrow[4 * (size_t)i + 3] = rsrc[2 * (size_t)o + 1];
+  }
   const size_t t_slot = blob.add(slot), t_mvp = blob.add(mv_ptr), t_mv = blob.add(mv),
                t_gp = blob.add(g_ptr), t_g = blob.add(g), t_in = blob.add(ineq), t_rs = blob.add(rsrc),
-               t_con = blob.add(con), t_pe = blob.add(pent), t_ae = blob.add(aent);
+               t_con = blob.add(con), t_pe = blob.add(pent), t_ae = blob.add(aent), t_mr = blob.add(mrow),
+               t_me = blob.add(ment);
   auto* plan = new wmpc_adjoint_plan();
   if (int rc = blob.upload(&plan->block, "adjoint plan")) {
     delete plan;
@@ -671,6 +712,9 @@ int wmpc_adjoint_plan_create(const wmpc_adjoint_desc* ds, const wmpc_qdldl_plan*
   d.con = blob.at<int4>(t_con);
   d.pent = blob.at<int2>(t_pe);
   d.aent = blob.at<int2>(t_ae);
+  d.nmv = (int32_t)(ment.size() / 4);
+  d.mrow = blob.at<int4>(t_mr);
+  d.ment = blob.at<int4>(t_me);
   plan->adj = adj;
   *out = plan;
   return WMPC_OK;
