@@ -100,6 +100,21 @@ struct TableStream {
     }
     return read_lane(cur, i - base);
   }
+  // Entries i .. i+3 (one range check when they lie in the current chunk).
+  __device__ __forceinline__ void get4(int32_t i, T& a, T& b, T& c, T& d) {
+    const int32_t j = i - base;
+    if (j + 3 < kWave) {
+      a = read_lane(cur, j);
+      b = read_lane(cur, j + 1);
+      c = read_lane(cur, j + 2);
+      d = read_lane(cur, j + 3);
+    } else {
+      a = (*this)[i];
+      b = (*this)[i + 1];
+      c = (*this)[i + 2];
+      d = (*this)[i + 3];
+    }
+  }
 };
 
 // LDS slots of the factorization (fused forward adds one window) and the
@@ -217,7 +232,8 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
       // live-L operands (LDS) come first, then the L entries re-read from
       // global memory, so neither loop branches on the operand's source
       for (; u + 4 <= pe.z; u += 4) {
-        const int2 e0 = upds[u], e1 = upds[u + 1], e2 = upds[u + 2], e3 = upds[u + 3];
+        int2 e0, e1, e2, e3;
+        upds.get4(u, e0, e1, e2, e3);
         const float l0 = AL[e0.y * lc], l1 = AL[e1.y * lc], l2 = AL[e2.y * lc], l3 = AL[e3.y * lc];
         const float y0 = Y[e0.x * lc], y1 = Y[e1.x * lc];
         const float y2 = Y[e2.x * lc], y3 = Y[e3.x * lc];
@@ -231,7 +247,8 @@ __device__ __forceinline__ void factor_rows(const QdldlProg& P, const Src& src, 
         Y[e.x * lc] -= AL[e.y * lc] * val;
       }
       for (; u + 4 <= pe.w; u += 4) {
-        const int2 e0 = upds[u], e1 = upds[u + 1], e2 = upds[u + 2], e3 = upds[u + 3];
+        int2 e0, e1, e2, e3;
+        upds.get4(u, e0, e1, e2, e3);
         const float l0 = lx(-(int64_t)e0.y - 1), l1 = lx(-(int64_t)e1.y - 1);
         const float l2 = lx(-(int64_t)e2.y - 1), l3 = lx(-(int64_t)e3.y - 1);
         const float y0 = Y[e0.x * lc], y1 = Y[e1.x * lc];
