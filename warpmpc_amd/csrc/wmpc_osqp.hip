@@ -84,14 +84,20 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
   float* Lx = Di + n;                                  // [nl + 1]
   int32_t* Li = reinterpret_cast<int32_t*>(Lx + nl + 1);  // [nl + 1]
   float* Kv = reinterpret_cast<float*>(Li + nl + 1);
+  // staging: unrolled so that each lane keeps several global loads in flight
+#pragma unroll 8
   for (int32_t i = lane; i < n; i += kWave) Rw[i] = F.rows[i];
   for (int32_t i = lane; i <= n; i += kWave) Y[i] = 0.f;
+#pragma unroll 8
   for (int32_t j = lane; j < F.ninit; j += kWave) In[j] = F.init[j];
+#pragma unroll 8
   for (int32_t q = lane; q <= nl; q += kWave) {
     Li[q] = q < nl ? F.li[q] : 0;
     Lx[q] = 0.f;
   }
+#pragma unroll 8
   for (int32_t r = lane; r < ns; r += kWave) St[r] = F.steps[r];
+#pragma unroll 8
   for (int32_t s = lane; s < F.nnz_kkt; s += kWave) Kv[s] = src(s, 0);
   __syncthreads();
   // this lane's update of a step: target row (n: the dummy row) and L value
