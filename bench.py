@@ -34,6 +34,11 @@ Extra legs on the same JSON line:
 * ``err_vs_fp64`` of each leg against the FP64 oracle on sampled instances;
 * ``cpu_baseline``: the oracle's float32 OSQP (OpenMP over instances) on a
   bounded sample of the headline workload, all host cores and one core.
+
+The oracle runs only on the CPU side, after every GPU leg has been timed: the
+GPU legs queue their accuracy checks with host copies of their sampled
+inputs and outputs, and the CPU-baseline leg runs them before it times the
+oracle itself.
 """
 
 from __future__ import annotations
@@ -144,6 +149,28 @@ def _err_summary(errs: dict) -> dict:
     return out
 
 
+# Accuracy checks against the FP64 oracle.  The oracle is test infrastructure:
+# the GPU legs only queue their checks (with host copies of their sampled
+# inputs and outputs), and the CPU-side leg runs them after every GPU leg has
+# been timed, next to the CPU baseline.  Each check fills dicts that the JSON
+# line already holds.
+_CHECKS: list = []
+
+
+def _check_later(fn) -> None:
+    """Queue fn(oracle_module, sqp_oracle_module)."""
+    _CHECKS.append(fn)
+
+
+def _run_checks() -> None:
+    from oracle import oracle, sqp_oracle
+
+    _log(f"accuracy checks against the oracle: {len(_CHECKS)}")
+    for fn in _CHECKS:
+        fn(oracle, sqp_oracle)
+    _CHECKS.clear()
+
+
 def _sym(t):
     """(B, w) batch-first -> contiguous (w, B) batch-major."""
     return t.t().contiguous()
@@ -182,7 +209,6 @@ def _u2_bytes(plan, k: int) -> int:
 def _osqp_leg(args, per_instance: bool, stream):
     import torch
 
-    from oracle import oracle
     from warpmpc_amd import _lib
     from warpmpc_amd.problems import batched_mpc_data
 
@@ -218,17 +244,21 @@ def _osqp_leg(args, per_instance: bool, stream):
     wall, dev_ms = _timed(step, args.steps, args.warmup, stream)
     # accuracy of every output against the FP64 oracle on sampled instances
     cols = np.random.default_rng(0).choice(b, size=min(256, b), replace=False)
-    orc = oracle.OracleOSQP(plan)
     p_in = pv[cols] if per_instance else pv
     a_in = av[cols] if per_instance else av
     p32 = np.asarray(p_in, dtype=np.float32).astype(np.float64)
     a32 = np.asarray(a_in, dtype=np.float32).astype(np.float64)
-    ref = orc.solve(p32, a32, q[cols].astype(np.float32), l[cols].astype(np.float32), u[cols].astype(np.float32))
+    data = (p32, a32, q[cols].astype(np.float32), l[cols].astype(np.float32), u[cols].astype(np.float32))
     got = [v[:, cols].t().cpu().numpy() for v in (x, z, y)]
-    errs = {k: _rel_err(g, r) for k, g, r in zip(("x", "z", "y"), got, ref[:3])}
+    errs = {}
+
+    def check(orc, _):
+        ref = orc.OracleOSQP(plan).solve(*data)
+        errs.update(_err_summary({k: _rel_err(g, r) for k, g, r in zip(("x", "z", "y"), got, ref[:3])}))
+
+    _check_later(check)
     kernel = f"wmpc_admm_shared_{dev.pattern_key():016x}" if dev.specialized and not per_instance else None
-    return dict(plan=plan, wall=wall, dev_ms=dev_ms, value=b * args.steps / wall, errs=_err_summary(errs),
-                kernel=kernel)
+    return dict(plan=plan, wall=wall, dev_ms=dev_ms, value=b * args.steps / wall, errs=errs, kernel=kernel)
 
 
 # --------------------------------------------------------------------------- fwd + adjoint (cart-pole)
@@ -238,7 +268,6 @@ def _adjoint_leg(args, stream, mode="reference"):
     import numpy as np
     import torch
 
-    from oracle import oracle
     from warpmpc_amd import _lib
     from warpmpc_amd.osqp import OSQPSettings, build_osqp_plan, compile_osqp
     from warpmpc_amd.problems import OSQP_INFTY, cartpole_dynamics, mpc_qp
@@ -320,22 +349,33 @@ def _adjoint_leg(args, stream, mode="reference"):
     u2 = _u2_bytes(plan, 25)
     out = dict(b=b, wall=wall, dev_ms=dev_ms, fwd_ms=fwd_ms, adj_ms=adj_ms, steps=steps, u2=u2, plan=plan, dp=dp,
                refine=refine, mode=mode)
+    errs, second = {}, {}
+    out["errs"] = errs
     if mode == "reference":
-        # the same adjoint in the reference's algorithm by the oracle, FP64 and FP32
-        adj = oracle.OracleAdjoint(plan, refine=refine)
-        r64 = adj.compute(*args_o)[3:]
-        r32 = adj.compute(*args_o, dtype=np.float32)[3:]
-        out["errs"] = _err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, r64)})
-        out["spread"] = _err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r32, r64)})
+        def check(orc, _):
+            # the same adjoint in the reference's algorithm by the oracle, FP64 and FP32
+            adj = orc.OracleAdjoint(plan, refine=refine)
+            r64 = adj.compute(*args_o)[3:]
+            r32 = adj.compute(*args_o, dtype=np.float32)[3:]
+            errs.update(_err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, r64)}))
+            second.update(_err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r32, r64)}))
+
+        out["spread"] = second
         # U3: B_qp + s [n + 2 (nnz_P + nnz_A) + (n + 2m) + R (2 nnz_L' + n_K') + F (nnz_K' + nnz_L' + n_K')]
         q2 = dp.qdldl_plan
     else:
-        # the active-set limit of the reference's refinement, FP64 (dense solve of M0)
-        ex = oracle.adjoint_exact(plan, *args_o, clamp_slacks=True)[3:]
-        out["errs"] = _err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, ex)})
-        r101 = oracle.OracleAdjoint(plan, refine=args.refine).compute(*args_o)[3:]
-        out["reference_algorithm_vs_limit"] = _err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r101, ex)})
+        ref_refine = args.refine
+
+        def check(orc, _):
+            # the active-set limit of the reference's refinement, FP64 (dense solve of M0)
+            ex = orc.adjoint_exact(plan, *args_o, clamp_slacks=True)[3:]
+            errs.update(_err_summary({k: _rel_err(g, r) for k, g, r in zip(names, got, ex)}))
+            r101 = orc.OracleAdjoint(plan, refine=ref_refine).compute(*args_o)[3:]
+            second.update(_err_summary({k: _rel_err(a, r) for k, a, r in zip(names, r101, ex)}))
+
+        out["reference_algorithm_vs_limit"] = second
         q2 = plan.qdldl_plan  # the forward KKT pattern
+    _check_later(check)
     adj_bytes = S * (n + 2 * (plan.nnz_p + plan.nnz_a) + (n + 2 * m) + (refine + 1) * (2 * q2.nnz_l + q2.n)
                      + 1 * (q2.nnz_a + q2.nnz_l + q2.n))
     out.update(u3=u2 + adj_bytes, adj_bytes=adj_bytes, q2=q2)
@@ -349,7 +389,6 @@ def _hetero_leg(args, stream):
     import numpy as np
     import torch
 
-    from oracle import oracle
     from warpmpc_amd.osqp import HeterogeneousOSQP, OSQPSettings, ShiftMaps, build_osqp_plan, compile_osqp, \
         shift_warm_start
     from warpmpc_amd.problems import batched_mpc_data, hetero_counts, hetero_mpc_patterns, mpc_shift_maps
@@ -382,15 +421,23 @@ def _hetero_leg(args, stream):
     wall, dev_ms = _timed(step, args.steps, args.warmup, stream)
     # accuracy: sampled instances of the last step against the oracle from the same warm start
     rng = np.random.default_rng(2)
-    errs = {"x": [], "z": [], "y": []}
+    samples = []
     for pl, d, o, w, c in zip(plans, data, state["out"], warm, counts):
         cols = rng.choice(int(c), size=16, replace=False)
         ws = [t[:, cols].t().cpu().numpy().astype(np.float64) for t in w]
-        ref = oracle.OracleOSQP(pl).solve(d[0], d[1], d[2][cols], d[3][cols], d[4][cols], x0=ws[0], z0=ws[1],
-                                          y0=ws[2])
-        for name, t, r in zip(("x", "z", "y"), o[:3], ref[:3]):
-            errs[name].append(_rel_err(t[:, cols].t().cpu().numpy(), r))
-    errs = {k: np.concatenate(v) for k, v in errs.items()}
+        got = [t[:, cols].t().cpu().numpy() for t in o[:3]]
+        samples.append((pl, (d[0], d[1], d[2][cols], d[3][cols], d[4][cols]), ws, got))
+    errs = {}
+
+    def check(orc, _):
+        acc = {"x": [], "z": [], "y": []}
+        for pl, d, ws, got in samples:
+            ref = orc.OracleOSQP(pl).solve(*d, x0=ws[0], z0=ws[1], y0=ws[2])
+            for name, g, r in zip(("x", "z", "y"), got, ref[:3]):
+                acc[name].append(_rel_err(g, r))
+        errs.update(_err_summary({k: np.concatenate(v) for k, v in acc.items()}))
+
+    _check_later(check)
     # U4: sum over patterns of U2 (plus the warm start's x0, z0, y0 reads) per instance
     u4 = sum(int(c) * (_u2_bytes(pl, 25) + S * (pl.n + 2 * pl.m)) for pl, c in zip(plans, counts))
     ach = u4 / (dev_ms * 1e-3) / 1e9
@@ -403,7 +450,7 @@ def _hetero_leg(args, stream):
                          "frac": ach / HBM_PEAK_GBS, "model_bytes_per_step": u4, "kernel": "k_osqp_hetero",
                          "traffic": _profile_value("osqp_hetero_nine_patterns_b200k_pmc.txt",
                                                    "traffic_bytes_per_launch")},
-            "err_vs_fp64": _err_summary(errs)}
+            "err_vs_fp64": errs}
 
 
 # --------------------------------------------------------------------------- nonlinear SQP (C4)
@@ -413,7 +460,6 @@ def _sqp_leg(args, stream):
     import numpy as np
     import torch
 
-    from oracle import sqp_oracle
     from warpmpc_amd.osqp import OSQPSettings
     from warpmpc_amd.sqp import build_sparse_mpc_plan, compile_sparse_mpc_sqp
     from warpmpc_amd.sqp.problems import QUAD_NU, QUAD_NX, quadrotor_initial_guess_and_params, quadrotor_problem
@@ -483,34 +529,40 @@ def _sqp_leg(args, stream):
     jac = labels()
     _, label_ms = _timed(labels, max(1, args.steps // 5), 1, stream)
     # the labels against the FP64 oracle of the same derivative (active-set KKT
-    # system at the same QP iterate) on sampled instances
-    from oracle import oracle as orc_mod
-
-    # An instance with every first-stage thrust at a bound has du0/dx0 = 0
-    # exactly (float64 gives ~1e-16, float32 ~1e-10), so each instance is
-    # measured against 1 % of its largest adjoint output (the bound rows' own
-    # derivative is 1 there) rather than against its Jacobian's norm.
+    # system at the same QP iterate) on sampled instances.  An instance with
+    # every first-stage thrust at a bound has du0/dx0 = 0 exactly (float64 gives
+    # ~1e-16, float32 ~1e-10), so each instance is measured against 1 % of its
+    # largest adjoint output (the bound rows' own derivative is 1 there) rather
+    # than against its Jacobian's norm.
     cols = np.random.default_rng(4).choice(b, size=128, replace=False)
     f64 = lambda t: t[cols].cpu().numpy().astype(np.float64)  # noqa: E731
-    ref = np.zeros((cols.size, QUAD_NU, QUAD_NX))
-    grad_scale = np.zeros(cols.size)
-    for k in range(QUAD_NU):
-        dx = np.zeros((cols.size, op.n))
-        dx[:, QUAD_NX + k] = 1.0
-        ex = orc_mod.adjoint_exact(op, f64(lin.p_values), f64(lin.a_values), f64(lin.l), f64(lin.u), f64(qp_out["d"]),
-                                   f64(qp_out["y"]), dx, np.zeros((cols.size, op.m)), clamp_slacks=True)
-        ref[:, k] = ex[4][:, :QUAD_NX] + ex[5][:, :QUAD_NX]
-        for r in ex[3:6]:
-            grad_scale = np.maximum(grad_scale, np.abs(r).reshape(cols.size, -1).max(1))
-    jac_err = _rel_err(jac[cols].cpu().numpy().reshape(cols.size, -1), ref.reshape(cols.size, -1),
-                       row_floor=1e-2 * grad_scale)
-    zero_jac = int(np.sum(np.abs(ref).reshape(cols.size, -1).max(1) < 1e-12))
-    # accuracy of one line-search step against the FP64 SQP oracle on a few instances
+    lab_in = (f64(lin.p_values), f64(lin.a_values), f64(lin.l), f64(lin.u), f64(qp_out["d"]), f64(qp_out["y"]))
+    jac_got = jac[cols].cpu().numpy().reshape(cols.size, -1)
+    # one line-search step on a few instances, for the FP64 SQP oracle
     zs, ps = z0[:4].cpu().numpy(), params[:4].cpu().numpy()
     res, _ = sqp.step(z0[:4], params[:4], state=sqp.init_state(4))
-    ref = sqp_oracle.OracleSQP(prob, plan).step(zs, ps)
-    err = _rel_err(res.solve.direction.cpu().numpy(), ref["direction"])
-    same_step = bool(np.array_equal(res.line_search.step_length.cpu().numpy(), ref["step_length"]))
+    step_dir = res.solve.direction.cpu().numpy()
+    step_len = res.line_search.step_length.cpu().numpy()
+    lab_err, first_step, lab_zero = {}, {}, {}
+
+    def check(orc, sqp_orc):
+        ref = np.zeros((cols.size, QUAD_NU, QUAD_NX))
+        grad_scale = np.zeros(cols.size)
+        for k in range(QUAD_NU):
+            dx = np.zeros((cols.size, op.n))
+            dx[:, QUAD_NX + k] = 1.0
+            ex = orc.adjoint_exact(op, *lab_in, dx, np.zeros((cols.size, op.m)), clamp_slacks=True)
+            ref[:, k] = ex[4][:, :QUAD_NX] + ex[5][:, :QUAD_NX]
+            for r in ex[3:6]:
+                grad_scale = np.maximum(grad_scale, np.abs(r).reshape(cols.size, -1).max(1))
+        jac_err = _rel_err(jac_got, ref.reshape(cols.size, -1), row_floor=1e-2 * grad_scale)
+        lab_err.update(_err_summary({"du0_dx0": jac_err}))
+        lab_zero["sampled_with_zero_jacobian"] = int(np.sum(np.abs(ref).reshape(cols.size, -1).max(1) < 1e-12))
+        sref = sqp_orc.OracleSQP(prob, plan).step(zs, ps)
+        first_step.update({"direction": float(np.max(_rel_err(step_dir, sref["direction"]))),
+                           "same_step_lengths": bool(np.array_equal(step_len, sref["step_length"]))})
+
+    _check_later(check)
     ms = wall * 1e3 / solves
     qp_ms = float(np.mean(phases["qp"]))
     u2 = _u2_bytes(op, 25) + S * (op.n + 2 * op.m)  # warm-started QP
@@ -523,10 +575,10 @@ def _sqp_leg(args, stream):
             "labels_du0_dx0": {"ms": label_ms, "per_s": b / (label_ms * 1e-3),
                                "method": "QP at the final iterate (25 ADMM iterations, warm start) + 4 KKT-form "
                                          "adjoint launches (one per input): du0/dx0 of the active-set KKT system",
-                               "err_vs_fp64": _err_summary({"du0_dx0": jac_err}),
+                               "err_vs_fp64": lab_err,
                                "err_measure": "per instance |J - J64|_inf / max(|J64|_inf, 1e-2 x the instance's "
                                               "largest adjoint output), 128 sampled instances",
-                               "sampled_with_zero_jacobian": zero_jac,
+                               "zero_jacobian": lab_zero,
                                "qp_prim_res_median": float(qp_out["prim"].median()),
                                "qp_dual_res_median": float(qp_out["dual"].median())},
             "config": "quadrotor (12 states, 4 thrusts, RK4 dt 0.02) N=25, x0 seed 4, hover initial guess, "
@@ -536,7 +588,7 @@ def _sqp_leg(args, stream):
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": ach / HBM_PEAK_GBS, "kernel": "QP phase: k_osqp_admm (per-instance factor) + layout copies",
                          "model_bytes_per_qp": u2, "kernel_ms": qp_ms},
-            "err_vs_fp64_first_step": {"direction": float(np.max(err)), "same_step_lengths": same_step}}
+            "err_vs_fp64_first_step": first_step}
 
 
 # --------------------------------------------------------------------------- QDLDL microbench (C1)
@@ -546,7 +598,6 @@ def _qdldl_leg(args, stream):
     import numpy as np
     import torch
 
-    from oracle import oracle
     from warpmpc_amd import _lib
     from warpmpc_amd.problems import kkt_microbench_plan, kkt_microbench_values
     from warpmpc_amd.qdldl import compile_qdldl_variant
@@ -575,9 +626,15 @@ def _qdldl_leg(args, stream):
     cols = np.random.default_rng(0).choice(b, size=min(256, b), replace=False)
     v64 = vals[:, cols].cpu().numpy().T.astype(np.float64)
     r64 = rhs[:, cols].cpu().numpy().T.astype(np.float64)
-    lx64, _, dinv64 = oracle.qdldl_factor(q, v64)
-    x64 = oracle.qdldl_solve(q, lx64, dinv64, r64)
-    rel = _rel_err(x[:, cols].cpu().numpy().T, x64)
+    x_got = x[:, cols].cpu().numpy().T
+    errs = {}
+
+    def check(orc, _):
+        lx64, _, dinv64 = orc.qdldl_factor(q, v64)
+        x64 = orc.qdldl_solve(q, lx64, dinv64, r64)
+        errs.update(_err_summary({"x": _rel_err(x_got, x64)}))
+
+    _check_later(check)
     # U1: read values + rhs; write L, D, Dinv; re-read L in the backward sweep; write x
     u1 = S * (q.nnz_a + 2 * q.nnz_l + 4 * q.n)
     ach = u1 * b / (dev_ms * 1e-3) / 1e9
@@ -588,7 +645,7 @@ def _qdldl_leg(args, stream):
                          "frac": ach / HBM_PEAK_GBS, "bytes_per_system": u1,
                          "traffic": _profile_value("qdldl_factor_solve_n50_b65536_pmc.txt",
                                                    "traffic_bytes_per_launch")},
-            "err_vs_fp64": _err_summary({"x": rel})}
+            "err_vs_fp64": errs}
 
 
 # --------------------------------------------------------------------------- CPU baseline
@@ -746,6 +803,8 @@ def main() -> int:
                                                    "traffic_bytes_per_launch")},
             "err_vs_fp64": r["errs"],
         }
+    # the CPU side: accuracy checks of the legs above, then the CPU baseline
+    _run_checks()
     if "cpu" in legs:
         if plan is None:
             _, _, plan = _chain_problem()
