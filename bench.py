@@ -586,7 +586,8 @@ def _sqp_leg(args, stream):
                       "the previous QP), each iteration one HIP graph replay; linearization by torch.func on the "
                       "GPU, QP by the HIP OSQP",
             "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": ach / HBM_PEAK_GBS, "kernel": "QP phase: k_osqp_admm (per-instance factor) + layout copies",
+                         "frac": ach / HBM_PEAK_GBS,
+                         "kernel": "QP phase: k_osqp_admm (per-instance factor) + layout copies",
                          "model_bytes_per_qp": u2, "kernel_ms": qp_ms},
             "err_vs_fp64_first_step": first_step}
 
