@@ -79,22 +79,22 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
   int4* Rw = reinterpret_cast<int4*>(sm);
   int2* In = reinterpret_cast<int2*>(sm + 4 * (int64_t)n);
   int2* St = In + F.ninit;
-  float* Y = reinterpret_cast<float*>(St + ns);  // [n + 1]
-  float* Di = Y + n + 1;
-  float* Lx = Di + n;                                  // [nl + 1]
-  int32_t* Li = reinterpret_cast<int32_t*>(Lx + nl + 1);  // [nl + 1]
-  float* Kv = reinterpret_cast<float*>(Li + nl + 1);
+  // row workspace and D⁻¹ interleaved, {Y[i], D⁻¹[i]} (n + 1 pairs: a dummy row
+  // for idle lanes), and L interleaved, {row li[q], value L[q]} (nl + 1 pairs):
+  // a step reads Y[c] with D⁻¹[c], and a lane its L row with its value, in one
+  // 8-byte LDS access
+  float* YD = reinterpret_cast<float*>(St + ns);
+  int32_t* LL = reinterpret_cast<int32_t*>(YD + 2 * ((int64_t)n + 1));
+  float* Kv = reinterpret_cast<float*>(LL + 2 * ((int64_t)nl + 1));
   // staging: unrolled so that each lane keeps several global loads in flight
 #pragma unroll 8
   for (int32_t i = lane; i < n; i += kWave) Rw[i] = F.rows[i];
-  for (int32_t i = lane; i <= n; i += kWave) Y[i] = 0.f;
+  for (int32_t i = lane; i <= n; i += kWave) reinterpret_cast<float2*>(YD)[i] = float2{0.f, 0.f};
 #pragma unroll 8
   for (int32_t j = lane; j < F.ninit; j += kWave) In[j] = F.init[j];
 #pragma unroll 8
-  for (int32_t q = lane; q <= nl; q += kWave) {
-    Li[q] = q < nl ? F.li[q] : 0;
-    Lx[q] = 0.f;
-  }
+  for (int32_t q = lane; q <= nl; q += kWave)
+    reinterpret_cast<int2*>(LL)[q] = int2{q < nl ? F.li[q] : 0, 0};
 #pragma unroll 8
   for (int32_t r = lane; r < ns; r += kWave) St[r] = F.steps[r];
 #pragma unroll 8
@@ -105,10 +105,9 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
     const int32_t cnt = rec.x >> 16;
     const bool on = lane < cnt;
     const int32_t q = rec.y + (on ? lane : 0);  // <= nnz_l (a padding slot)
-    const int32_t li = Li[q];
-    const float l = Lx[q];
-    row = on ? li : n;
-    lv = on ? l : 0.f;
+    const int2 e = reinterpret_cast<const int2*>(LL)[q];
+    row = on ? e.x : n;
+    lv = on ? __int_as_float(e.y) : 0.f;
   };
   const int32_t slast = ns > 0 ? ns - 1 : 0;
   int2 rec_a = ns > 0 ? St[0] : int2{0, 0};
@@ -118,7 +117,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
     const int4 row = Rw[k];  // {end of steps, init begin, init end, diagonal slot}
     for (int32_t j = row.y + lane; j < row.z; j += kWave) {
       const int2 e = In[j];
-      Y[e.x] = Kv[e.y];
+      YD[2 * e.x] = Kv[e.y];
     }
     float dk = Kv[row.w];
     int32_t ia = n;
@@ -133,27 +132,28 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
       lane_update(rec_b, ia, la);  // step r + 1 (its row is k or starts afresh)
       rec_a = rec_b;
       rec_b = St[r + 2 < ns ? r + 2 : slast];
-      const float yc = Y[c];
-      const float dc = Di[c];
-      const float yi = Y[i0];
-      Y[i0] = yi - l0 * yc;
+      const float2 ycd = reinterpret_cast<const float2*>(YD)[c];
+      const float yc = ycd.x, dc = ycd.y;
+      const float yi = YD[2 * i0];
+      YD[2 * i0] = yi - l0 * yc;
       if (cnt > kWave) {
         for (int32_t t = lane + kWave; t < cnt; t += kWave) {
           const int32_t q = q0 + t;
-          Y[Li[q]] -= Lx[q] * yc;
+          const int2 e = reinterpret_cast<const int2*>(LL)[q];
+          YD[2 * e.x] -= __int_as_float(e.y) * yc;
         }
       }
       const float lval = yc * dc;
-      Lx[q0 + cnt] = lval;  // every lane stores the same value
-      Y[c] = 0.f;
+      LL[2 * (q0 + cnt) + 1] = __float_as_int(lval);  // every lane stores the same value
+      YD[2 * c] = 0.f;
       dk -= yc * lval;
     }
     const float di = 1.0f / dk;
-    Di[k] = di;
+    YD[2 * k + 1] = di;
     if (lane == 0) dinv[k] = di;
   }
   __syncthreads();
-  for (int32_t q = lane; q < nl; q += kWave) lx[q] = Lx[q];
+  for (int32_t q = lane; q < nl; q += kWave) lx[q] = __int_as_float(LL[2 * q + 1]);
 }
 
 static size_t factor_lds(const QdldlProg& P) {
