@@ -102,35 +102,78 @@ __device__ __forceinline__ void admm_pass(const OsqpDev& dv, const CStrided& S, 
                                           const CStrided& Q, const CStrided& Lo,
                                           const CStrided& Up) {
   const float alpha = dv.alpha, oma = 1.0f - dv.alpha, sigma = dv.sigma;
+  // entries per step: each entry reads and writes only its own variable or
+  // constraint, so the operand loads of kU entries are issued before any of
+  // their stores (several round trips in flight instead of one)
+  constexpr int kU = 8;
   TableStream<int4> vt(dv.vtab, dv.nvar, 0);
-  for (int32_t v = 0; v < dv.nvar; ++v) {
-    const int4 t = vt[v];
-    const float xo = X(t.y);
+  auto var = [&](const int4 t, float xo, float sx, float qv) {
     float xn = xo;
     if constexpr (!FIRST) {
-      xn = alpha * S(t.x) + oma * xo;
+      xn = alpha * sx + oma * xo;
       X(t.y) = xn;
     }
-    if constexpr (!LAST) R(t.z) = sigma * xn - __int_as_float(t.w) * Q(t.y);
+    if constexpr (!LAST) R(t.z) = sigma * xn - __int_as_float(t.w) * qv;
+  };
+  int32_t v = 0;
+  for (; v + kU <= dv.nvar; v += kU) {
+    int4 t[kU];
+    float xo[kU], sx[kU], qv[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) t[i] = vt[v + i];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      xo[i] = X(t[i].y);
+      sx[i] = FIRST ? 0.f : S(t[i].x);
+      qv[i] = LAST ? 0.f : Q(t[i].y);
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) var(t[i], xo[i], sx[i], qv[i]);
+  }
+  for (; v < dv.nvar; ++v) {
+    const int4 t = vt[v];
+    var(t, X(t.y), FIRST ? 0.f : S(t.x), LAST ? 0.f : Q(t.y));
   }
   TableStream<int4> ct(dv.ctab, 2 * dv.ncon, 0);
-  for (int32_t k = 0; k < dv.ncon; ++k) {
-    const int4 t = ct[2 * k];
+  auto con = [&](const int4 t, const int4 t2, float zo, float yo, float sx, float lo, float up) {
     const float rinv = __int_as_float(t.w);
-    const float zo = Z(t.y), yo = Y(t.y);
     float zn = zo, yn = yo;
     if constexpr (!FIRST) {
-      const int4 t2 = ct[2 * k + 1];
       const float rho = __int_as_float(t2.x), e = __int_as_float(t2.y);
       const float rz = zo - rinv * yo;
-      const float zt = rz + rinv * S(t.x);
+      const float zt = rz + rinv * sx;
       const float zr = alpha * zt + oma * zo;
-      zn = nan_min(nan_max(zr + rinv * yo, Lo(t.y) * e), Up(t.y) * e);
+      zn = nan_min(nan_max(zr + rinv * yo, lo * e), up * e);
       yn = yo + rho * (zr - zn);
       Z(t.y) = zn;
       Y(t.y) = yn;
     }
     if constexpr (!LAST) R(t.z) = zn - rinv * yn;
+  };
+  int32_t k = 0;
+  for (; k + kU <= dv.ncon; k += kU) {
+    int4 t[kU], t2[kU];
+    float zo[kU], yo[kU], sx[kU], lo[kU], up[kU];
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      t[i] = ct[2 * (k + i)];
+      t2[i] = FIRST ? t[i] : ct[2 * (k + i) + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) {
+      zo[i] = Z(t[i].y);
+      yo[i] = Y(t[i].y);
+      sx[i] = FIRST ? 0.f : S(t[i].x);
+      lo[i] = FIRST ? 0.f : Lo(t[i].y);
+      up[i] = FIRST ? 0.f : Up(t[i].y);
+    }
+#pragma unroll
+    for (int i = 0; i < kU; ++i) con(t[i], t2[i], zo[i], yo[i], sx[i], lo[i], up[i]);
+  }
+  for (; k < dv.ncon; ++k) {
+    const int4 t = ct[2 * k];
+    const int4 t2 = FIRST ? t : ct[2 * k + 1];
+    con(t, t2, Z(t.y), Y(t.y), FIRST ? 0.f : S(t.x), FIRST ? 0.f : Lo(t.y), FIRST ? 0.f : Up(t.y));
   }
 }
 
