@@ -209,6 +209,19 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint(
     for (int32_t i = 0; i < dv.dim; ++i) {
       const int4 r = rows[i];
       float s = 0.0f;
+      // four entries' operands loaded before they are summed in order
+      for (; e + 4 <= r.y; e += 4) {
+        int4 t[4];
+        float v[4], x[4];
+        ents.get4(e, t[0], t[1], t[2], t[3]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v[j] = m0.entry(t[j]);
+          x[j] = SOL(t[j].x & 0x0FFFFFFF);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s += v[j] * x[j];
+      }
       for (; e < r.y; ++e) {
         const int4 t = ents[e];
         s += m0.entry(t) * SOL(t.x & 0x0FFFFFFF);
@@ -420,6 +433,19 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
     for (int32_t i = 0; i < dv.nk; ++i) {
       const int2 r = rows[i];
       float s = 0.0f;
+      // four entries' operands loaded before they are summed in order
+      for (; e + 4 <= r.y; e += 4) {
+        int4 t[4];
+        float v[4], w[4];
+        ents.get4(e, t[0], t[1], t[2], t[3]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          v[j] = m0.entry(t[j], r.x);
+          w[j] = W(t[j].x);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s += v[j] * w[j];
+      }
       for (; e < r.y; ++e) {
         const int4 t = ents[e];
         s += m0.entry(t, r.x) * W(t.x);
