@@ -201,35 +201,72 @@ __device__ __forceinline__ void admm_finish(const OsqpDev& dv, const float* __re
   const int64_t pb = pm_ld ? b : 0;
   const int64_t pes = pm_ld ? pm_ld : 1;  // entry stride of P/A
   const int32_t n = dv.n, m = dv.m;
-  for (int32_t i = 0; i < n; ++i) X(i) *= dv.sd[i];
-  for (int32_t i = 0; i < m; ++i) {
-    Z(i) *= dv.seinv[i];
-    Y(i) *= dv.y_unscale[i];
-  }
-  float pr = 0.0f;
-  for (int32_t i = 0; i < m; ++i) {
-    float s = 0.0f;
-    for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
-      const int2 t = dv.arow[e];
-      s += av[(int64_t)t.y * pes + pb] * X(t.x);
+  // Tables are read as streams (every lane takes part) and each loop takes
+  // several entries per step, their loads issued before their stores or
+  // before the in-order sums that consume them.
+  auto unscale = [&](const Strided& V, const float* f, int32_t cnt) {
+    TableStream<float> fs(f, cnt, 0);
+    int32_t i = 0;
+    for (; i + 8 <= cnt; i += 8) {
+      float v[8], g[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        v[j] = V(i + j);
+        g[j] = fs[i + j];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) V(i + j) = v[j] * g[j];
     }
-    pr = nan_max(pr, fabsf(s - Z(i)));
+    for (; i < cnt; ++i) V(i) *= fs[i];
+  };
+  unscale(X, dv.sd, n);
+  unscale(Z, dv.seinv, m);
+  unscale(Y, dv.y_unscale, m);
+  // sum over one row's entries [e, end) of vals[slot] * V(col), in order
+  auto row_sum = [&](TableStream<int2>& ents, int32_t& e, int32_t end, const float* vals,
+                     const Strided& V) {
+    float s = 0.0f;
+    for (; e + 4 <= end; e += 4) {
+      int2 t[4];
+      float v[4], x[4];
+      ents.get4(e, t[0], t[1], t[2], t[3]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v[j] = vals[(int64_t)t[j].y * pes + pb];
+        x[j] = V(t[j].x);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) s += v[j] * x[j];
+    }
+    for (; e < end; ++e) {
+      const int2 t = ents[e];
+      s += vals[(int64_t)t.y * pes + pb] * V(t.x);
+    }
+    return s;
+  };
+  float pr = 0.0f;
+  {
+    TableStream<int32_t> ends(dv.arow_ptr + 1, m, 0);
+    TableStream<int2> ents(dv.arow, dv.arow_ptr[m], 0);
+    int32_t e = dv.arow_ptr[0];
+    for (int32_t i = 0; i < m; ++i) {
+      const float s = row_sum(ents, e, ends[i], av, X);
+      pr = nan_max(pr, fabsf(s - Z(i)));
+    }
   }
   float du = 0.0f, quad = 0.0f, lin = 0.0f;
-  for (int32_t j = 0; j < n; ++j) {
-    float px = 0.0f, aty = 0.0f;
-    for (int32_t e = dv.prow_ptr[j]; e < dv.prow_ptr[j + 1]; ++e) {
-      const int2 t = dv.prow[e];
-      px += pv[(int64_t)t.y * pes + pb] * X(t.x);
+  {
+    TableStream<int32_t> pends(dv.prow_ptr + 1, n, 0), aends(dv.acol_ptr + 1, n, 0);
+    TableStream<int2> pents(dv.prow, dv.prow_ptr[n], 0), aents(dv.acol, dv.acol_ptr[n], 0);
+    int32_t ep = dv.prow_ptr[0], ea = dv.acol_ptr[0];
+    for (int32_t j = 0; j < n; ++j) {
+      const float px = row_sum(pents, ep, pends[j], pv, X);
+      const float aty = row_sum(aents, ea, aends[j], av, Y);
+      const float xj = X(j), qj = Q(j);
+      du = nan_max(du, fabsf(px + qj + aty));
+      quad += xj * px;
+      lin += qj * xj;
     }
-    for (int32_t e = dv.acol_ptr[j]; e < dv.acol_ptr[j + 1]; ++e) {
-      const int2 t = dv.acol[e];
-      aty += av[(int64_t)t.y * pes + pb] * Y(t.x);
-    }
-    const float xj = X(j), qj = Q(j);
-    du = nan_max(du, fabsf(px + qj + aty));
-    quad += xj * px;
-    lin += qj * xj;
   }
   prim[b] = pr;
   dual[b] = du;

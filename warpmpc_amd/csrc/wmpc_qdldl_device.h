@@ -64,6 +64,12 @@ struct QdldlProg {
   int32_t ninit = 0, nproc = 0, nupd = 0;
 };
 
+__device__ __forceinline__ int32_t read_lane(int32_t v, int32_t j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+__device__ __forceinline__ float read_lane(float v, int32_t j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
 __device__ __forceinline__ int2 read_lane(int2 v, int32_t j) {
   return make_int2(__builtin_amdgcn_readlane(v.x, j), __builtin_amdgcn_readlane(v.y, j));
 }
