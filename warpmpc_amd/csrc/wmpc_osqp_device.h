@@ -183,11 +183,29 @@ __device__ __forceinline__ void admm_initial(const OsqpDev& dv, const float* __r
                                              const float* __restrict__ y0, const Strided& X,
                                              const Strided& Z, const Strided& Y, int64_t ld,
                                              int64_t b) {
-  for (int32_t i = 0; i < dv.n; ++i) X(i) = x0 ? x0[(int64_t)i * ld + b] * dv.sdinv[i] : 0.0f;
-  for (int32_t i = 0; i < dv.m; ++i) {
-    Z(i) = z0 ? z0[(int64_t)i * ld + b] * dv.se[i] : 0.0f;
-    Y(i) = y0 ? y0[(int64_t)i * ld + b] * dv.y_scale[i] : 0.0f;
-  }
+  // V(i) = src[i] * f[i] (0 without src), 8 entries per step, loads first
+  auto scaled = [&](const Strided& V, const float* __restrict__ src, const float* f, int32_t cnt) {
+    if (!src) {
+      for (int32_t i = 0; i < cnt; ++i) V(i) = 0.0f;
+      return;
+    }
+    TableStream<float> fs(f, cnt, 0);
+    int32_t i = 0;
+    for (; i + 8 <= cnt; i += 8) {
+      float v[8], g[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        v[j] = src[(int64_t)(i + j) * ld + b];
+        g[j] = fs[i + j];
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) V(i + j) = v[j] * g[j];
+    }
+    for (; i < cnt; ++i) V(i) = src[(int64_t)i * ld + b] * fs[i];
+  };
+  scaled(X, x0, dv.sdinv, dv.n);
+  scaled(Z, z0, dv.se, dv.m);
+  scaled(Y, y0, dv.y_scale, dv.m);
 }
 
 // Unscaling (solver.py:863-865), then the residuals and the objective with the
