@@ -362,14 +362,32 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
   const uint32_t pl = pm_ld ? (uint32_t)b : 0u;
   const CStrided X{x, ld, b}, Y{y, ld, b}, Lo{l, ld, b}, Up{u, ld, b}, DY{dy, ld, b};
   const Strided DG{dg, ld, b}, RC{rc, ld, b}, AX{axv, ld, b}, W{w, ld, b}, RB{rbuf, ld, b};
-  // 1. per A row: A x, the coupling diagonal and the right-hand side
+  // 1. per A row: A x, the coupling diagonal and the right-hand side (tables
+  // as streams; a row's operands loaded four at a time, summed in order)
+  TableStream<int32_t> aends(dv.arow_ptr + 1, dv.m, 0);
+  TableStream<int2> aents(dv.arow, dv.arow_ptr[dv.m], 0);
+  TableStream<int4> cons(dv.con, dv.m, 0);
+  int32_t ae = dv.arow_ptr[0];
   for (int32_t i = 0; i < dv.m; ++i) {
     float ax = 0.0f;
-    for (int32_t e = dv.arow_ptr[i]; e < dv.arow_ptr[i + 1]; ++e) {
-      const int2 t = dv.arow[e];
-      ax += (av + (int64_t)t.y * pes)[pl] * X(t.x);
+    const int32_t aend = aends[i];
+    for (; ae + 4 <= aend; ae += 4) {
+      int2 ta[4];
+      float va[4], xa[4];
+      aents.get4(ae, ta[0], ta[1], ta[2], ta[3]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        va[j] = (av + (int64_t)ta[j].y * pes)[pl];
+        xa[j] = X(ta[j].x);
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) ax += va[j] * xa[j];
     }
-    const int4 t = dv.con[i];
+    for (; ae < aend; ++ae) {
+      const int2 ta = aents[ae];
+      ax += (av + (int64_t)ta.y * pes)[pl] * X(ta.x);
+    }
+    const int4 t = cons[i];
     const float yv = Y(i), dyv = DY(i);
     const float lu = nan_max(yv, 0.0f), ll = -nan_min(yv, 0.0f);
     float d = kDecoupled, r = 0.0f;
@@ -514,16 +532,51 @@ __global__ __launch_bounds__(kWave) void k_osqp_adjoint_kkt(
       }
     }
   }
+  // dP and dA, 8 entries per step with their loads first (only real lanes
+  // run here, so the entry tables are read directly, not as streams)
   if (dp) {
     const Strided DP{dp, ld, b};
-    for (int32_t k = 0; k < dv.nnz_p; ++k) {
+    int32_t k = 0;
+    for (; k + 8 <= dv.nnz_p; k += 8) {
+      int2 t[8];
+      float wx[8], xy[8], wy[8], xx[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t[j] = dv.pent[k + j];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        wx[j] = W(t[j].x);
+        xy[j] = X(t[j].y);
+        wy[j] = W(t[j].y);
+        xx[j] = X(t[j].x);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) DP(k + j) = 0.5f * (wx[j] * xy[j] + wy[j] * xx[j]);
+    }
+    for (; k < dv.nnz_p; ++k) {
       const int2 t = dv.pent[k];
       DP(k) = 0.5f * (W(t.x) * X(t.y) + W(t.y) * X(t.x));
     }
   }
   if (da) {
     const Strided DA{da, ld, b};
-    for (int32_t k = 0; k < dv.nnz_a; ++k) {
+    int32_t k = 0;
+    for (; k + 8 <= dv.nnz_a; k += 8) {
+      int2 t[8];
+      float yx[8], wy[8], ux[8], lxv[8], xy[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t[j] = dv.aent[k + j];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        yx[j] = Y(t[j].x);
+        wy[j] = W(t[j].y);
+        ux[j] = DU(t[j].x);
+        lxv[j] = DL(t[j].x);
+        xy[j] = X(t[j].y);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) DA(k + j) = yx[j] * wy[j] + (-ux[j] - lxv[j]) * xy[j];
+    }
+    for (; k < dv.nnz_a; ++k) {
       const int2 t = dv.aent[k];
       DA(k) = Y(t.x) * W(t.y) + (-DU(t.x) - DL(t.x)) * X(t.y);
     }
