@@ -129,12 +129,15 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
       const int32_t c = sx & 0xffff, cnt = sx >> 16;
       const int32_t i0 = ia;
       const float l0 = la;
-      lane_update(rec_b, ia, la);  // step r + 1 (its row is k or starts afresh)
-      rec_a = rec_b;
-      rec_b = St[r + 2 < ns ? r + 2 : slast];
       const float2 ycd = reinterpret_cast<const float2*>(YD)[c];
       const float yc = ycd.x, dc = ycd.y;
       const float yi = YD[2 * i0];
+      // issued after this step's Y reads so that the wait for those does not
+      // also wait for these; step r's L write goes to column c's slot, which
+      // step r + 1 (another column) does not read
+      lane_update(rec_b, ia, la);  // step r + 1 (its row is k or starts afresh)
+      rec_a = rec_b;
+      rec_b = St[r + 2 < ns ? r + 2 : slast];
       YD[2 * i0] = yi - l0 * yc;
       if (cnt > kWave) {
         for (int32_t t = lane + kWave; t < cnt; t += kWave) {
