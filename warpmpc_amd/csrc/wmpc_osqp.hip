@@ -62,9 +62,9 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
 // processing order the updates Y[li[q]] -= L[q] Y[c] over the column's entries
 // before row k -- distinct rows, one lane each -- and L[k,c] = Y[c] D⁻¹[c],
 // D[k] -= Y[c] L[k,c].  Every value sees the same operations in the same order
-// as in the one-lane factorization.  All tables and values live in LDS (the
-// KKT values are assembled there by all lanes first), so the loop touches no
-// global memory until L is written out.  A step's record is read two steps
+// as in the one-lane factorization.  All tables and values live in LDS (all
+// lanes first assemble the KKT values into the row and init records), so the
+// loop touches no global memory until L is written out.  A step's record is read two steps
 // ahead and each lane's L entry and target row one step ahead (L entries of a
 // reach column before row k were written in earlier rows; at the start of a
 // row they are read again, since the row just finished may have written them);
@@ -85,20 +85,28 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
   // 8-byte LDS access
   float* YD = reinterpret_cast<float*>(St + ns);
   int32_t* LL = reinterpret_cast<int32_t*>(YD + 2 * ((int64_t)n + 1));
-  float* Kv = reinterpret_cast<float*>(LL + 2 * ((int64_t)nl + 1));
-  // staging: unrolled so that each lane keeps several global loads in flight
+  // staging: unrolled so that each lane keeps several global loads in flight.
+  // The KKT values are resolved here: a row record keeps its diagonal value and
+  // an init record its entry's value (as bits) in place of the KKT slot, so a
+  // row's scatter reads each entry with one LDS access.
 #pragma unroll 8
-  for (int32_t i = lane; i < n; i += kWave) Rw[i] = F.rows[i];
+  for (int32_t i = lane; i < n; i += kWave) {
+    int4 rw = F.rows[i];
+    rw.w = __float_as_int(src(rw.w, 0));
+    Rw[i] = rw;
+  }
   for (int32_t i = lane; i <= n; i += kWave) reinterpret_cast<float2*>(YD)[i] = float2{0.f, 0.f};
 #pragma unroll 8
-  for (int32_t j = lane; j < F.ninit; j += kWave) In[j] = F.init[j];
+  for (int32_t j = lane; j < F.ninit; j += kWave) {
+    int2 e = F.init[j];
+    e.y = __float_as_int(src(e.y, 0));
+    In[j] = e;
+  }
 #pragma unroll 8
   for (int32_t q = lane; q <= nl; q += kWave)
     reinterpret_cast<int2*>(LL)[q] = int2{q < nl ? F.li[q] : 0, 0};
 #pragma unroll 8
   for (int32_t r = lane; r < ns; r += kWave) St[r] = F.steps[r];
-#pragma unroll 8
-  for (int32_t s = lane; s < F.nnz_kkt; s += kWave) Kv[s] = src(s, 0);
   __syncthreads();
   // this lane's update of a step: target row (n: the dummy row) and L value
   auto lane_update = [&](int2 rec, int32_t& row, float& lv) {
@@ -114,12 +122,12 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
   int2 rec_b = ns > 0 ? St[ns > 1 ? 1 : 0] : int2{0, 0};
   int32_t r = 0;
   for (int32_t k = 0; k < n; ++k) {
-    const int4 row = Rw[k];  // {end of steps, init begin, init end, diagonal slot}
+    const int4 row = Rw[k];  // {end of steps, init begin, init end, diagonal value bits}
     for (int32_t j = row.y + lane; j < row.z; j += kWave) {
-      const int2 e = In[j];
-      YD[2 * e.x] = Kv[e.y];
+      const int2 e = In[j];  // {row, value bits}
+      YD[2 * e.x] = __int_as_float(e.y);
     }
-    float dk = Kv[row.w];
+    float dk = __int_as_float(row.w);
     int32_t ia = n;
     float la = 0.f;
     if (ns > 0) lane_update(rec_a, ia, la);

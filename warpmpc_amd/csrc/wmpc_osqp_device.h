@@ -57,12 +57,12 @@ struct SharedFactorProg {
 
 // LDS layout of k_osqp_factor_shared (float offsets): rows (4n), init entries
 // (2 ninit), steps (2 nsteps), row workspace and D⁻¹ interleaved (2 (n + 1): a
-// dummy row for idle lanes), L rows and values interleaved (2 (nnz_l + 1)),
-// assembled KKT values (nnz_kkt).  The 16- and 8-byte records come first, so
-// they stay aligned.
+// dummy row for idle lanes), L rows and values interleaved (2 (nnz_l + 1)).
+// The KKT values live in the row and init records.  The 16- and 8-byte records
+// come first, so they stay aligned.
 __host__ __device__ inline int64_t shared_factor_lds_floats(const SharedFactorProg& F) {
   return 4 * (int64_t)F.n + 2 * (int64_t)F.ninit + 2 * (int64_t)F.nsteps + 2 * ((int64_t)F.n + 1) +
-         2 * ((int64_t)F.nnz_l + 1) + (int64_t)F.nnz_kkt;
+         2 * ((int64_t)F.nnz_l + 1);
 }
 inline size_t shared_factor_lds(const SharedFactorProg& F) {
   return (size_t)shared_factor_lds_floats(F) * sizeof(float);
