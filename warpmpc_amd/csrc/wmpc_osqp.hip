@@ -64,8 +64,8 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
 // D[k] -= Y[c] L[k,c].  Every value sees the same operations in the same order
 // as in the one-lane factorization.  All tables and values live in LDS (all
 // lanes first assemble the KKT values into the row and init records), so the
-// loop touches no global memory until L is written out.  A step's record is read two steps
-// ahead and each lane's L entry and target row one step ahead (L entries of a
+// loop touches no global memory until L is written out.  A step's record is
+// read two steps ahead and each lane's L entry and target row one step ahead (L entries of a
 // reach column before row k were written in earlier rows; at the start of a
 // row they are read again, since the row just finished may have written them);
 // lanes past a step's update count work on a dummy row.  The wavefront's LDS
