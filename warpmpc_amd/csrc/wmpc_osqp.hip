@@ -65,9 +65,10 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
 // as in the one-lane factorization.  All tables and values live in LDS (all
 // lanes first assemble the KKT values into the row and init records), so the
 // loop touches no global memory until L is written out.  A step's record is
-// read two steps ahead and each lane's L entry and target row one step ahead (L entries of a
-// reach column before row k were written in earlier rows; at the start of a
-// row they are read again, since the row just finished may have written them);
+// read two steps ahead and each lane's L entry and target row one step ahead
+// (L entries of a reach column before row k were written in earlier rows; at
+// the start of a row they are read again, since the row just finished may have
+// written them);
 // lanes past a step's update count work on a dummy row.  The wavefront's LDS
 // accesses execute in program order, so dependent steps need no barrier.
 __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F, KktSource src,
@@ -135,9 +136,12 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F
       const int32_t sx = __builtin_amdgcn_readfirstlane(rec_a.x);
       const int32_t q0 = __builtin_amdgcn_readfirstlane(rec_a.y);
       const int32_t c = sx & 0xffff, cnt = sx >> 16;
+      // the same column index from the record's vector copy (equal on every
+      // lane): its address skips the vector-to-scalar round trip
+      const int32_t cv = rec_a.x & 0xffff;
       const int32_t i0 = ia;
       const float l0 = la;
-      const float2 ycd = reinterpret_cast<const float2*>(YD)[c];
+      const float2 ycd = reinterpret_cast<const float2*>(YD)[cv];
       const float yc = ycd.x, dc = ycd.y;
       const float yi = YD[2 * i0];
       // issued after this step's Y reads so that the wait for those does not
