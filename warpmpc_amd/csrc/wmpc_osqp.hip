@@ -68,8 +68,7 @@ __global__ __launch_bounds__(kWave) void k_osqp_factor(QdldlProg P, KktSource sr
 // read two steps ahead and each lane's L entry and target row one step ahead
 // (L entries of a reach column before row k were written in earlier rows; at
 // the start of a row they are read again, since the row just finished may have
-// written them);
-// lanes past a step's update count work on a dummy row.  The wavefront's LDS
+// written them); lanes past a step's update count work on a dummy row.  The wavefront's LDS
 // accesses execute in program order, so dependent steps need no barrier.
 __global__ __launch_bounds__(kWave) void k_osqp_factor_shared(SharedFactorProg F, KktSource src,
                                                               float* __restrict__ lx,
